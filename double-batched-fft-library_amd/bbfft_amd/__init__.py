@@ -1,0 +1,321 @@
+"""Python mirror of the bbfft C++ API over the gfx950 C ABI (include/bbfft_hip.h).
+
+Names follow the reference's C++ interface (include/bbfft/configuration.hpp,
+include/bbfft/parser.hpp, include/bbfft/plan.hpp) so tests read like the reference's tests:
+
+    cfg = parse_fft_descriptor("scfo8.256*32768")
+    plan = make_plan(cfg)                # current torch stream, current device
+    plan.execute(x, y)                   # device pointers or torch tensors; asynchronous
+
+PyTorch (when present) is only used for device pointers and the current stream.
+"""
+import ctypes
+import enum
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+from . import _capi
+
+__all__ = [
+    "precision", "direction", "transform_type", "kernel_language", "configuration",
+    "user_module", "parse_fft_descriptor", "default_istride", "default_ostride", "make_plan",
+    "Plan", "Cache", "BadConfiguration", "BbfftError", "DescriptorError", "kernel_configuration",
+    "generate_source", "aot_generate", "device_info_query", "MAX_FFT_DIM", "MAX_TENSOR_DIM",
+]
+
+MAX_FFT_DIM = 3
+MAX_TENSOR_DIM = MAX_FFT_DIM + 2
+
+
+class precision(enum.IntEnum):
+    f32 = 4
+    f64 = 8
+
+
+class direction(enum.IntEnum):
+    forward = -1
+    backward = 1
+
+
+class transform_type(enum.IntEnum):
+    c2c = 0
+    r2c = 1
+    c2r = 2
+
+
+class kernel_language(enum.IntEnum):
+    opencl_c = 0
+    hip = 1
+
+
+class BbfftError(RuntimeError):
+    """HIP runtime / hiprtc failure or internal error."""
+
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+class BadConfiguration(BbfftError):
+    """Unsupported or faulty configuration (bbfft::bad_configuration)."""
+
+
+class DescriptorError(BbfftError):
+    """Malformed descriptor (std::runtime_error from the parser)."""
+
+
+def _check(status):
+    if status != _capi.SUCCESS:
+        msg = _capi.load().bbfft_last_error().decode(errors="replace")
+        if status == _capi.ERROR_BAD_CONFIGURATION:
+            raise BadConfiguration(status, msg)
+        if status == _capi.ERROR_PARSE:
+            raise DescriptorError(status, msg)
+        if status == _capi.ERROR_INVALID_ARGUMENT:
+            raise ValueError(msg)
+        raise BbfftError(status, msg)
+
+
+@dataclass
+class user_module:
+    """Load/store callbacks (include/bbfft/user_module.hpp of the reference)."""
+    data: Optional[str] = None
+    load_function: Optional[str] = None
+    store_function: Optional[str] = None
+    language: kernel_language = kernel_language.hip
+
+    def __bool__(self):
+        return bool(self.data) and (self.load_function is not None
+                                    or self.store_function is not None)
+
+
+def _arr5(v):
+    v = list(v) + [0] * (5 - len(v))
+    return (ctypes.c_size_t * 5)(*v[:5])
+
+
+def default_istride(dim, shape, type_, inplace):
+    out = (ctypes.c_size_t * 5)()
+    _check(_capi.load().bbfft_default_istride(dim, _arr5(shape), int(type_), int(inplace), out))
+    return list(out)
+
+
+def default_ostride(dim, shape, type_, inplace):
+    out = (ctypes.c_size_t * 5)()
+    _check(_capi.load().bbfft_default_ostride(dim, _arr5(shape), int(type_), int(inplace), out))
+    return list(out)
+
+
+@dataclass
+class configuration:
+    """Mirror of bbfft::configuration; strides default to the in-place layout like the
+    reference's default member initialisers (include/bbfft/configuration.hpp:168-179)."""
+    dim: int
+    shape: Sequence[int]
+    fp: precision = precision.f32
+    dir: direction = direction.forward
+    type: transform_type = transform_type.c2c
+    istride: Optional[Sequence[int]] = None
+    ostride: Optional[Sequence[int]] = None
+    callbacks: user_module = field(default_factory=user_module)
+
+    def __post_init__(self):
+        self.shape = list(self.shape) + [0] * (5 - len(self.shape))
+        if self.istride is None:
+            self.istride = default_istride(self.dim, self.shape, self.type, True)
+        if self.ostride is None:
+            self.ostride = default_ostride(self.dim, self.shape, self.type, True)
+        self.istride = list(self.istride) + [0] * (5 - len(self.istride))
+        self.ostride = list(self.ostride) + [0] * (5 - len(self.ostride))
+
+    def set_strides_default(self, inplace):
+        self.istride = default_istride(self.dim, self.shape, self.type, inplace)
+        self.ostride = default_ostride(self.dim, self.shape, self.type, inplace)
+
+    def _to_c(self):
+        c = _capi.CConfiguration()
+        c.dim = self.dim
+        c.shape = _arr5(self.shape)
+        c.precision = int(self.fp)
+        c.direction = int(self.dir)
+        c.type = int(self.type)
+        c.istride = _arr5(self.istride)
+        c.ostride = _arr5(self.ostride)
+        cb = self.callbacks
+        if cb:
+            src = cb.data.encode()
+            self._keep = src  # keep the buffer alive while the C struct is in use
+            c.callback_source = src
+            c.callback_length = len(src)
+            c.load_function = cb.load_function.encode() if cb.load_function else None
+            c.store_function = cb.store_function.encode() if cb.store_function else None
+            c.callback_language = int(cb.language)
+        return c
+
+    def to_string(self):
+        buf = ctypes.create_string_buffer(256)
+        _check(_capi.load().bbfft_configuration_to_string(ctypes.byref(self._to_c()), buf, 256))
+        return buf.value.decode()
+
+    def __str__(self):
+        return self.to_string()
+
+
+def parse_fft_descriptor(desc: str) -> configuration:
+    c = _capi.CConfiguration()
+    _check(_capi.load().bbfft_parse_fft_descriptor(desc.encode(), ctypes.byref(c)))
+    return configuration(dim=c.dim, shape=list(c.shape), fp=precision(c.precision),
+                         dir=direction(c.direction), type=transform_type(c.type),
+                         istride=list(c.istride), ostride=list(c.ostride))
+
+
+def normalize_device_info(desc: str) -> str:
+    buf = ctypes.create_string_buffer(256)
+    _check(_capi.load().bbfft_device_info_normalize(desc.encode(), buf, 256))
+    return buf.value.decode()
+
+
+GFX950_INFO = "{1024, {64}, 65536, gpu}"
+
+
+def num_passes(cfg: configuration) -> int:
+    n = ctypes.c_uint()
+    _check(_capi.load().bbfft_num_passes(ctypes.byref(cfg._to_c()), ctypes.byref(n)))
+    return n.value
+
+
+def kernel_configuration(cfg: configuration, pass_index=0, device_info=GFX950_INFO):
+    """Tiling of the kernel of 1-D pass `pass_index` (dict) plus its identifier."""
+    kc = _capi.CKernelConfig()
+    name = ctypes.create_string_buffer(512)
+    _check(_capi.load().bbfft_kernel_configuration(ctypes.byref(cfg._to_c()), pass_index,
+                                                   device_info.encode(), ctypes.byref(kc), name,
+                                                   512))
+    d = {f: getattr(kc, f) for f, _ in _capi.CKernelConfig._fields_ if f != "factorization"}
+    d["factorization"] = list(kc.factorization)[:kc.num_factors]
+    d["name"] = name.value.decode()
+    return d
+
+
+def generate_source(cfg: configuration, pass_index=0, device_info=GFX950_INFO) -> str:
+    L = _capi.load()
+    c = cfg._to_c()
+    n = ctypes.c_size_t()
+    _check(L.bbfft_generate_source(ctypes.byref(c), pass_index, device_info.encode(), None, 0,
+                                   ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value)
+    _check(L.bbfft_generate_source(ctypes.byref(c), pass_index, device_info.encode(), buf,
+                                   n.value, None))
+    return buf.value.decode()
+
+
+def aot_generate(descriptors: List[str], out_path: str, device_info=GFX950_INFO,
+                 arch="gfx950", jobs=8) -> int:
+    arr = (ctypes.c_char_p * len(descriptors))(*[d.encode() for d in descriptors])
+    n = ctypes.c_size_t()
+    _check(_capi.load().bbfft_aot_generate(arr, len(descriptors), device_info.encode(),
+                                           arch.encode(), out_path.encode(), jobs,
+                                           ctypes.byref(n)))
+    return n.value
+
+
+def device_info_query(device=0) -> str:
+    buf = ctypes.create_string_buffer(256)
+    _check(_capi.load().bbfft_device_info_query(device, buf, 256))
+    return buf.value.decode()
+
+
+class Cache:
+    """jit_cache_all layered over an aot_cache (see bbfft_cache_create)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        _check(_capi.load().bbfft_cache_create(ctypes.byref(self._h)))
+
+    def load_aot(self, path, device=0):
+        n = ctypes.c_size_t()
+        _check(_capi.load().bbfft_cache_load_aot(self._h, path.encode(), device, ctypes.byref(n)))
+        return n.value
+
+    def __len__(self):
+        n = ctypes.c_size_t()
+        _check(_capi.load().bbfft_cache_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _capi.load().bbfft_cache_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    raise TypeError("expected a device pointer (int) or a tensor with data_ptr()")
+
+
+def _current_stream():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return torch.cuda.current_stream().cuda_stream
+    except ImportError:
+        pass
+    return 0
+
+
+class Plan:
+    """Mirror of plan_unmanaged_event<hipEvent_t> (include/bbfft/plan.hpp:144-184)."""
+
+    def __init__(self, handle, cfg, stream):
+        self._h = handle
+        self.cfg = cfg
+        self.stream = stream
+
+    def execute(self, inp, out=None, signal=None, wait: Sequence = ()):
+        """Out-of-place if out is given, else in-place. Events are hipEvent_t handles (int)."""
+        src = _ptr(inp)
+        dst = src if out is None else _ptr(out)
+        nw = len(wait)
+        warr = (ctypes.c_void_p * max(1, nw))(*[int(w) for w in wait]) if nw else None
+        _check(_capi.load().bbfft_plan_execute(self._h, src, dst, signal, nw, warr))
+
+    def launches(self):
+        L = _capi.load()
+        n = ctypes.c_uint()
+        _check(L.bbfft_plan_num_launches(self._h, ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            name = ctypes.create_string_buffer(512)
+            g, b, lds = ctypes.c_uint(), ctypes.c_uint(), ctypes.c_size_t()
+            _check(L.bbfft_plan_launch_info(self._h, i, name, 512, ctypes.byref(g),
+                                            ctypes.byref(b), ctypes.byref(lds)))
+            out.append({"kernel": name.value.decode(), "grid": g.value, "block": b.value,
+                        "lds_bytes": lds.value})
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _capi.load().bbfft_plan_destroy(self._h)
+            self._h = None
+
+
+def make_plan(cfg, stream=None, cache: Optional[Cache] = None) -> Plan:
+    """make_plan(cfg, queue, cache) for the current HIP device; cfg may be a descriptor."""
+    if isinstance(cfg, str):
+        cfg = parse_fft_descriptor(cfg)
+    if stream is None:
+        stream = _current_stream()
+    h = ctypes.c_void_p()
+    _check(_capi.load().bbfft_plan_create(ctypes.byref(cfg._to_c()), stream,
+                                          cache.handle if cache is not None else None,
+                                          ctypes.byref(h)))
+    return Plan(h, cfg, stream)

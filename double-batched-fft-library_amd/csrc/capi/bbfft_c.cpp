@@ -1,0 +1,427 @@
+// C ABI over the C++ API (declarations and reference citations in include/bbfft_hip.h).
+#include "bbfft_hip.h"
+
+#include "../base/math.hpp"
+#include "../hip/aot_archive.hpp"
+#include "../hip/hip_plan_impl.hpp"
+#include "../plan/decompose.hpp"
+#include "bbfft/aot_cache.hpp"
+#include "bbfft/bad_configuration.hpp"
+#include "bbfft/configuration.hpp"
+#include "bbfft/detail/generator_impl.hpp"
+#include "bbfft/hip/device.hpp"
+#include "bbfft/hip/error.hpp"
+#include "bbfft/hip/make_plan.hpp"
+#include "bbfft/hip/online_compiler.hpp"
+#include "bbfft/jit_cache_all.hpp"
+#include "bbfft/parser.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace bbfft;
+
+struct bbfft_plan_s {
+    std::shared_ptr<hip_detail::plan_base> impl;
+};
+
+/// AOT modules first, then modules compiled in this process.
+struct bbfft_cache_s : public jit_cache {
+    aot_cache aot;
+    jit_cache_all jit;
+    std::size_t aot_kernels = 0;
+    auto get(jit_cache_key const &key) const -> shared_handle<module_handle_t> override {
+        auto m = aot.get(key);
+        return m ? m : jit.get(key);
+    }
+    void store(jit_cache_key const &key, shared_handle<module_handle_t> mod) override {
+        jit.store(key, std::move(mod));
+    }
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F> int guarded(F &&f) {
+    try {
+        g_last_error.clear();
+        f();
+        return BBFFT_SUCCESS;
+    } catch (bad_configuration const &e) {
+        g_last_error = e.what();
+        return BBFFT_ERROR_BAD_CONFIGURATION;
+    } catch (hip::error const &e) {
+        g_last_error = e.what();
+        return BBFFT_ERROR_RUNTIME;
+    } catch (std::invalid_argument const &e) {
+        g_last_error = e.what();
+        return BBFFT_ERROR_INVALID_ARGUMENT;
+    } catch (std::runtime_error const &e) {
+        g_last_error = e.what();
+        return BBFFT_ERROR_PARSE;
+    } catch (std::exception const &e) {
+        g_last_error = e.what();
+        return BBFFT_ERROR_INTERNAL;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return BBFFT_ERROR_INTERNAL;
+    }
+}
+
+void require(bool ok, char const *what) {
+    if (!ok) {
+        throw std::invalid_argument(what);
+    }
+}
+
+configuration from_c(bbfft_configuration const &c) {
+    configuration cfg = {};
+    cfg.dim = c.dim;
+    for (unsigned i = 0; i < max_tensor_dim; ++i) {
+        cfg.shape[i] = c.shape[i];
+        cfg.istride[i] = c.istride[i];
+        cfg.ostride[i] = c.ostride[i];
+    }
+    require(c.precision == 4 || c.precision == 8, "precision must be 4 or 8");
+    require(c.direction == -1 || c.direction == 1, "direction must be -1 or +1");
+    require(c.type >= 0 && c.type <= 2, "type must be 0 (c2c), 1 (r2c) or 2 (c2r)");
+    cfg.fp = static_cast<precision>(c.precision);
+    cfg.dir = static_cast<direction>(c.direction);
+    cfg.type = static_cast<transform_type>(c.type);
+    cfg.callbacks.data = c.callback_source;
+    cfg.callbacks.length = c.callback_length;
+    cfg.callbacks.load_function = c.load_function;
+    cfg.callbacks.store_function = c.store_function;
+    cfg.callbacks.language =
+        c.callback_language == 1 ? kernel_language::hip : kernel_language::opencl_c;
+    return cfg;
+}
+
+void to_c(configuration const &cfg, bbfft_configuration &c) {
+    std::memset(&c, 0, sizeof(c));
+    c.dim = cfg.dim;
+    for (unsigned i = 0; i < max_tensor_dim; ++i) {
+        c.shape[i] = cfg.shape[i];
+        c.istride[i] = cfg.istride[i];
+        c.ostride[i] = cfg.ostride[i];
+    }
+    c.precision = static_cast<int>(cfg.fp);
+    c.direction = static_cast<int>(cfg.dir);
+    c.type = static_cast<int>(cfg.type);
+}
+
+void copy_out(std::string const &s, char *buf, std::size_t len) {
+    require(buf != nullptr || len == 0, "null output buffer");
+    if (s.size() + 1 > len) {
+        throw std::invalid_argument("output buffer too small (" + std::to_string(s.size() + 1) +
+                                    " bytes needed)");
+    }
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+}
+
+std::array<std::size_t, max_tensor_dim> shape_of(size_t const *shape) {
+    std::array<std::size_t, max_tensor_dim> s{};
+    for (unsigned i = 0; i < max_tensor_dim; ++i) {
+        s[i] = shape[i];
+    }
+    return s;
+}
+
+fft1d_configuration kernel_config_of(bbfft_configuration const *cfg, unsigned pass,
+                                     char const *device_info, configuration *pass_cfg) {
+    require(cfg && device_info, "null argument");
+    auto c = from_c(*cfg);
+    auto layout = decompose(c);
+    require(pass < layout.passes.size(), "pass index out of range");
+    if (pass_cfg) {
+        *pass_cfg = layout.passes[pass];
+    }
+    return configure_fft1d(layout.passes[pass], parse_device_info(device_info));
+}
+
+} // namespace
+
+extern "C" {
+
+const char *bbfft_last_error(void) { return g_last_error.c_str(); }
+
+const char *bbfft_version(void) { return "0.5.1-gfx950"; }
+
+int bbfft_parse_fft_descriptor(const char *desc, bbfft_configuration *cfg) {
+    return guarded([&] {
+        require(desc && cfg, "null argument");
+        to_c(parse_fft_descriptor(desc), *cfg);
+    });
+}
+
+int bbfft_configuration_to_string(const bbfft_configuration *cfg, char *buf, size_t len) {
+    return guarded([&] {
+        require(cfg != nullptr, "null configuration");
+        copy_out(from_c(*cfg).to_string(), buf, len);
+    });
+}
+
+int bbfft_default_istride(unsigned dim, const size_t shape[5], int type, int inplace,
+                          size_t stride[5]) {
+    return guarded([&] {
+        require(shape && stride && type >= 0 && type <= 2, "invalid argument");
+        auto s = default_istride(dim, shape_of(shape), static_cast<transform_type>(type), inplace);
+        std::copy(s.begin(), s.end(), stride);
+    });
+}
+
+int bbfft_default_ostride(unsigned dim, const size_t shape[5], int type, int inplace,
+                          size_t stride[5]) {
+    return guarded([&] {
+        require(shape && stride && type >= 0 && type <= 2, "invalid argument");
+        auto s = default_ostride(dim, shape_of(shape), static_cast<transform_type>(type), inplace);
+        std::copy(s.begin(), s.end(), stride);
+    });
+}
+
+int bbfft_device_info_normalize(const char *desc, char *buf, size_t len) {
+    return guarded([&] {
+        require(desc != nullptr, "null descriptor");
+        copy_out(parse_device_info(desc).to_string(), buf, len);
+    });
+}
+
+int bbfft_factor(unsigned n, unsigned index, unsigned *factors) {
+    return guarded([&] {
+        require(factors != nullptr || index == 0, "null output");
+        auto f = factor(n, index);
+        std::copy(f.begin(), f.end(), factors);
+    });
+}
+
+unsigned bbfft_iroot(unsigned radicand, unsigned index) { return iroot(radicand, index); }
+
+int bbfft_is_prime(unsigned n) { return is_prime(n) ? 1 : 0; }
+
+int bbfft_num_passes(const bbfft_configuration *cfg, unsigned *num) {
+    return guarded([&] {
+        require(cfg && num, "null argument");
+        *num = static_cast<unsigned>(decompose(from_c(*cfg)).passes.size());
+    });
+}
+
+int bbfft_kernel_configuration(const bbfft_configuration *cfg, unsigned pass,
+                               const char *device_info, bbfft_kernel_config *out, char *name,
+                               size_t name_len) {
+    return guarded([&] {
+        require(out != nullptr, "null output");
+        auto kc = kernel_config_of(cfg, pass, device_info, nullptr);
+        std::memset(out, 0, sizeof(*out));
+        out->M = kc.M;
+        out->Mb = kc.Mb;
+        out->N = kc.N;
+        out->Nt = kc.Nt;
+        out->Kb = kc.Kb;
+        out->lds_pad = kc.lds_pad;
+        out->lds_bytes = kc.lds_bytes();
+        out->work_group_size = kc.work_group_size();
+        out->num_factors = static_cast<int>(std::min<std::size_t>(8, kc.factorization.size()));
+        for (int i = 0; i < out->num_factors; ++i) {
+            out->factorization[i] = kc.factorization[i];
+        }
+        out->inplace_unsupported = kc.inplace_unsupported ? 1 : 0;
+        if (name) {
+            copy_out(kc.identifier(), name, name_len);
+        }
+    });
+}
+
+int bbfft_generate_source(const bbfft_configuration *cfg, unsigned pass, const char *device_info,
+                          char *buf, size_t len, size_t *needed) {
+    return guarded([&] {
+        configuration pc = {};
+        auto kc = kernel_config_of(cfg, pass, device_info, &pc);
+        std::ostringstream os;
+        generate_fft1d(os, kc);
+        auto s = os.str();
+        if (needed) {
+            *needed = s.size() + 1;
+        }
+        if (buf) {
+            copy_out(s, buf, len);
+        }
+    });
+}
+
+int bbfft_cache_create(bbfft_cache_t *cache) {
+    return guarded([&] {
+        require(cache != nullptr, "null output");
+        *cache = new bbfft_cache_s();
+    });
+}
+
+int bbfft_cache_load_aot(bbfft_cache_t cache, const char *path, int device, size_t *num) {
+    return guarded([&] {
+        require(cache && path, "null argument");
+        auto n = hip::register_aot_archive(cache->aot, path, device);
+        cache->aot_kernels += n;
+        if (num) {
+            *num = n;
+        }
+    });
+}
+
+int bbfft_cache_size(bbfft_cache_t cache, size_t *num) {
+    return guarded([&] {
+        require(cache && num, "null argument");
+        *num = cache->jit.kernel_names().size() + cache->aot_kernels;
+    });
+}
+
+void bbfft_cache_destroy(bbfft_cache_t cache) { delete cache; }
+
+int bbfft_plan_create(const bbfft_configuration *cfg, bbfft_stream_t stream, bbfft_cache_t cache,
+                      bbfft_plan_t *plan) {
+    return guarded([&] {
+        require(cfg && plan, "null argument");
+        auto impl = hip_detail::create(from_c(*cfg), static_cast<hipStream_t>(stream), cache);
+        *plan = new bbfft_plan_s{std::move(impl)};
+    });
+}
+
+int bbfft_plan_create_from_descriptor(const char *desc, bbfft_stream_t stream,
+                                      bbfft_cache_t cache, bbfft_plan_t *plan) {
+    return guarded([&] {
+        require(desc && plan, "null argument");
+        auto impl =
+            hip_detail::create(parse_fft_descriptor(desc), static_cast<hipStream_t>(stream), cache);
+        *plan = new bbfft_plan_s{std::move(impl)};
+    });
+}
+
+int bbfft_plan_execute(bbfft_plan_t plan, const void *in, void *out, bbfft_event_t signal,
+                       uint32_t num_wait, const bbfft_event_t *wait) {
+    return guarded([&] {
+        require(plan != nullptr, "null plan");
+        require(num_wait == 0 || wait != nullptr, "wait events missing");
+        std::vector<hipEvent_t> w(num_wait);
+        for (uint32_t i = 0; i < num_wait; ++i) {
+            w[i] = static_cast<hipEvent_t>(wait[i]);
+        }
+        plan->impl->execute(in, out, static_cast<hipEvent_t>(signal), num_wait,
+                            num_wait ? w.data() : nullptr);
+    });
+}
+
+int bbfft_plan_num_launches(bbfft_plan_t plan, unsigned *num) {
+    return guarded([&] {
+        require(plan && num, "null argument");
+        *num = static_cast<unsigned>(plan->impl->launches().size());
+    });
+}
+
+int bbfft_plan_launch_info(bbfft_plan_t plan, unsigned index, char *name, size_t name_len,
+                           unsigned *grid, unsigned *block, size_t *lds_bytes) {
+    return guarded([&] {
+        require(plan != nullptr, "null plan");
+        auto l = plan->impl->launches();
+        require(index < l.size(), "launch index out of range");
+        if (name) {
+            copy_out(l[index].kernel_name, name, name_len);
+        }
+        if (grid) {
+            *grid = l[index].grid;
+        }
+        if (block) {
+            *block = l[index].block;
+        }
+        if (lds_bytes) {
+            *lds_bytes = l[index].lds_bytes;
+        }
+    });
+}
+
+void bbfft_plan_destroy(bbfft_plan_t plan) { delete plan; }
+
+int bbfft_device_info_query(int device, char *buf, size_t len) {
+    return guarded([&] { copy_out(hip::get_device_info(device).to_string(), buf, len); });
+}
+
+int bbfft_aot_generate(const char *const *descriptors, size_t num_descriptors,
+                       const char *device_info, const char *arch, const char *out_path, int jobs,
+                       size_t *num_kernels) {
+    return guarded([&] {
+        require(descriptors && device_info && arch && out_path, "null argument");
+        auto info = parse_device_info(device_info);
+        // Unique kernels in first-seen order.
+        std::vector<std::pair<std::string, std::string>> kernels; // (name, source)
+        std::set<std::string> seen;
+        for (size_t i = 0; i < num_descriptors; ++i) {
+            auto cfg = parse_fft_descriptor(descriptors[i]);
+            for (auto const &p : decompose(cfg).passes) {
+                auto kc = configure_fft1d(p, info);
+                auto id = kc.identifier();
+                if (seen.insert(id).second) {
+                    std::ostringstream os;
+                    generate_fft1d(os, kc);
+                    kernels.emplace_back(id, os.str());
+                }
+            }
+        }
+        // Several kernels per code object keeps the archive small and hiprtc calls few; chunks
+        // compile in parallel.
+        std::size_t const per_object = 8;
+        std::size_t const nobj = (kernels.size() + per_object - 1) / per_object;
+        std::vector<hip::archive_entry> entries(nobj);
+        std::atomic<std::size_t> next{0};
+        std::mutex err_mtx;
+        std::string first_error;
+        auto worker = [&] {
+            for (;;) {
+                std::size_t o = next++;
+                if (o >= nobj) {
+                    return;
+                }
+                std::string src;
+                auto &e = entries[o];
+                e.arch = arch;
+                for (std::size_t k = o * per_object;
+                     k < std::min(kernels.size(), (o + 1) * per_object); ++k) {
+                    src += kernels[k].second;
+                    e.kernel_names.push_back(kernels[k].first);
+                }
+                try {
+                    e.code = hip::compile_to_native(src, arch);
+                } catch (std::exception const &ex) {
+                    std::lock_guard<std::mutex> lock(err_mtx);
+                    if (first_error.empty()) {
+                        first_error = ex.what();
+                    }
+                }
+            }
+        };
+        int nt = std::max(1, std::min<int>(jobs, static_cast<int>(nobj)));
+        std::vector<std::thread> threads;
+        for (int t = 1; t < nt; ++t) {
+            threads.emplace_back(worker);
+        }
+        worker();
+        for (auto &t : threads) {
+            t.join();
+        }
+        if (!first_error.empty()) {
+            throw hip::error(first_error, 0);
+        }
+        hip::write_archive(out_path, entries);
+        if (num_kernels) {
+            *num_kernels = kernels.size();
+        }
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,198 @@
+// Tiling and factorisation heuristics for the gfx950 kernels.
+// Plays the role of the reference's configure_small_batch_fft / configure_factor2_slm_fft
+// (src/base/generator/small_batch_fft.cpp:15-58, factor2_slm_fft.cpp:16-86), re-derived for
+// wave64, 160 KiB LDS and Stockham stages exchanged through LDS.
+#include "../base/math.hpp"
+#include "bbfft/bad_configuration.hpp"
+#include "bbfft/detail/generator_impl.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <sstream>
+
+namespace bbfft {
+
+std::size_t fft1d_configuration::N_fft() const {
+    bool real = type != transform_type::c2c;
+    return real && N % 2 == 0 ? N / 2 : N;
+}
+
+bool fft1d_configuration::pairs_k() const { return type != transform_type::c2c && N % 2 == 1; }
+
+std::size_t fft1d_configuration::K_columns(std::size_t K) const {
+    return pairs_k() ? (K + 1) / 2 : K;
+}
+
+namespace {
+std::size_t padded(std::size_t n, std::size_t pad) { return pad ? n + n / pad : n; }
+
+bool needs_lds(fft1d_configuration const &c) {
+    return c.factorization.size() > 1 || c.type != transform_type::c2c;
+}
+} // namespace
+
+std::size_t fft1d_configuration::lds_bytes() const {
+    if (!needs_lds(*this)) {
+        return 0;
+    }
+    std::size_t np = padded(N_fft() - 1, lds_pad) + 1;
+    return Mb * Kb * np * 2 * static_cast<std::size_t>(fp);
+}
+
+std::string fft1d_configuration::identifier() const {
+    std::ostringstream oss;
+    oss << "bbfft_" << (direction < 0 ? 'm' : 'p') << (direction < 0 ? -direction : direction)
+        << "_M" << M << "_Mb" << Mb << "_N" << N << "_f";
+    for (std::size_t i = 0; i < factorization.size(); ++i) {
+        oss << (i ? "x" : "") << factorization[i];
+    }
+    oss << "_Nt" << Nt << "_Kb" << Kb << "_f" << static_cast<int>(fp) * 8 << '_'
+        << to_string(type) << "_is" << istride[0] << '_' << istride[1] << '_' << istride[2]
+        << "_os" << ostride[0] << '_' << ostride[1] << '_' << ostride[2] << "_p" << lds_pad
+        << "_in" << inplace_unsupported;
+    if (load_function) {
+        oss << '_' << load_function;
+    }
+    if (store_function) {
+        oss << '_' << store_function;
+    }
+    return oss.str();
+}
+
+namespace {
+
+/// Stage radices for a complex FFT of length n: one stage up to `single_max`, otherwise the
+/// fewest stages whose largest radix does not exceed `radix_max` (factors closest to n^(1/L)).
+std::vector<int> default_factorization(std::size_t n, std::size_t single_max,
+                                       std::size_t radix_max) {
+    if (n <= single_max) {
+        return {static_cast<int>(n)};
+    }
+    for (unsigned L = 2; L <= 4; ++L) {
+        auto f = factor(static_cast<unsigned>(n), L);
+        // drop unit factors
+        std::vector<int> g;
+        for (auto v : f) {
+            if (v > 1) {
+                g.push_back(static_cast<int>(v));
+            }
+        }
+        int fmax = *std::max_element(g.begin(), g.end());
+        if (static_cast<std::size_t>(fmax) <= radix_max || L == 4) {
+            // Largest radix first: the first stage then has the fewest butterflies, which sets
+            // the work-item count, and later stages loop over more butterflies per work-item.
+            std::sort(g.begin(), g.end(), std::greater<int>());
+            return g;
+        }
+    }
+    return {static_cast<int>(n)};
+}
+
+} // namespace
+
+fft1d_configuration configure_fft1d(configuration const &cfg, device_info const &info,
+                                    fft1d_tuning const &tuning) {
+    if (cfg.dim != 1) {
+        throw bad_configuration("configure_fft1d requires a 1-D configuration");
+    }
+    if (cfg.istride[0] != 1 || cfg.ostride[0] != 1) {
+        throw bad_configuration("Stride s_0 != 1 is not supported");
+    }
+    if ((cfg.type == transform_type::r2c && cfg.dir != direction::forward) ||
+        (cfg.type == transform_type::c2r && cfg.dir != direction::backward)) {
+        throw bad_configuration("r2c must be forward and c2r must be backward");
+    }
+    std::size_t const M = cfg.shape[0], N = cfg.shape[1], K = cfg.shape[2];
+    if (M == 0 || N == 0 || K == 0) {
+        throw bad_configuration("Tensor shape must not contain zeros");
+    }
+
+    fft1d_configuration c = {};
+    c.direction = static_cast<int>(cfg.dir);
+    c.M = M;
+    c.N = N;
+    c.fp = cfg.fp;
+    c.type = cfg.type;
+    c.istride = {cfg.istride[0], cfg.istride[1], cfg.istride[2]};
+    c.ostride = {cfg.ostride[0], cfg.ostride[1], cfg.ostride[2]};
+    c.load_function = cfg.callbacks.load_function;
+    c.store_function = cfg.callbacks.store_function;
+    std::size_t const nfft = c.N_fft();
+    std::size_t const real_bytes = static_cast<std::size_t>(cfg.fp);
+
+    if (!tuning.factorization.empty()) {
+        std::size_t p = 1;
+        for (auto f : tuning.factorization) {
+            if (f < 1) {
+                throw bad_configuration("factorization entries must be positive");
+            }
+            p *= static_cast<std::size_t>(f);
+        }
+        if (p != nfft) {
+            throw bad_configuration("factorization does not multiply to the FFT length");
+        }
+        c.factorization = tuning.factorization;
+    } else {
+        std::size_t single_max = real_bytes == 4 ? 16 : 16;
+        std::size_t radix_max = real_bytes == 4 ? 32 : 16;
+        c.factorization = default_factorization(nfft, single_max, radix_max);
+    }
+    for (auto f : c.factorization) {
+        if (f > 64 && f != 1) {
+            throw bad_configuration("Radix " + std::to_string(f) +
+                                    " exceeds the in-register limit (64); N = " +
+                                    std::to_string(N) + " is not supported");
+        }
+    }
+    int const rmax = *std::max_element(c.factorization.begin(), c.factorization.end());
+    c.Nt = tuning.Nt ? tuning.Nt : nfft / static_cast<std::size_t>(rmax);
+
+    std::size_t const wg_max = std::min<std::size_t>(1024, info.max_work_group_size);
+    if (c.Nt == 0 || c.Nt > wg_max) {
+        throw bad_configuration("Invalid number of work-items per transform");
+    }
+    if (tuning.Mb) {
+        c.Mb = std::min(tuning.Mb, M);
+    } else {
+        std::size_t mmax = std::max<std::size_t>(1, std::min<std::size_t>(64, 512 / c.Nt));
+        std::size_t mg = (M + mmax - 1) / mmax;
+        c.Mb = (M + mg - 1) / mg;
+    }
+    if (c.Mb * c.Nt > wg_max) {
+        throw bad_configuration("M tile times work-items per transform exceeds work-group size");
+    }
+
+    c.lds_pad = 0;
+    if (tuning.lds_pad) {
+        c.lds_pad = tuning.lds_pad;
+    } else if (!tuning.no_pad && (c.factorization.size() > 1 || cfg.type != transform_type::c2c)) {
+        c.lds_pad = nfft >= 32 ? 16 : 0;
+    }
+    if (c.lds_pad & (c.lds_pad - 1)) {
+        throw bad_configuration("lds_pad must be a power of two");
+    }
+
+    std::size_t const kcols = c.K_columns(K);
+    std::size_t const lds_limit = std::min<std::size_t>(info.local_memory_size, 64 * 1024);
+    if (tuning.Kb) {
+        c.Kb = tuning.Kb;
+    } else {
+        std::size_t target = 256;
+        std::size_t kb = std::max<std::size_t>(1, target / (c.Mb * c.Nt));
+        kb = std::min(kb, kcols);
+        c.Kb = kb;
+        while (c.Kb > 1 && c.lds_bytes() > lds_limit) {
+            --c.Kb;
+        }
+    }
+    if (c.work_group_size() > wg_max) {
+        throw bad_configuration("Work-group size exceeds the device limit");
+    }
+    if (c.lds_bytes() > info.local_memory_size) {
+        throw bad_configuration("Work-group exceeds the local memory size");
+    }
+    c.inplace_unsupported = cfg.type != transform_type::c2c && c.Mb < M;
+    return c;
+}
+
+} // namespace bbfft

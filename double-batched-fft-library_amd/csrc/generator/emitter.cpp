@@ -1,0 +1,701 @@
+// HIP source generator for gfx950 double-batched 1-D FFT kernels.
+//
+// Replaces the reference's clir/OpenCL-C generators (src/base/generator/sbfft_gen.cpp:30-351,
+// f2fft_gen.cpp:46-502, mixed_radix_fft.cpp:25-266). The kernel structure is designed for CDNA:
+//   * work-item (ml, t, kl) of a Mb x Nt x Kb work-group; the Mb M-columns are the fastest index
+//     so a wave's global accesses are contiguous along the stride-1 M mode;
+//   * Stockham autosort stages: stage s with radix R reads x[j + i N/R] and writes
+//     y[(j / p) p R + j % p + i p] (p = product of earlier radices); the first stage reads global
+//     memory and the last stage writes global memory, both coalesced across j; intermediate
+//     stages go through padded LDS;
+//   * radix-R butterflies are straight-line register code: Cooley-Tukey splits with radix-4/2
+//     kernels, symmetric-pair odd-prime kernels and constant twiddles simplified at generation;
+//   * real transforms of even length use the half-length complex FFT with a fused LDS pre/post
+//     pass (reference: sbfft_gen.cpp:180-247); odd lengths pack two K-columns into one complex FFT
+//     (reference: sbfft_gen.cpp:249-351);
+//   * callbacks are plain HIP device functions called with element offsets
+//     (reference: src/base/generator/tensor_accessor.cpp:39-50).
+#include "../base/math.hpp"
+#include "bbfft/bad_configuration.hpp"
+#include "bbfft/detail/generator_impl.hpp"
+#include "roots.hpp"
+
+#include <cstdio>
+#include <ostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace bbfft {
+
+namespace {
+
+/// Line-oriented code writer with fresh-name generation.
+class writer {
+  public:
+    void line(std::string const &s) { os_ << std::string(2 * indent_, ' ') << s << '\n'; }
+    void open(std::string const &s) {
+        line(s + " {");
+        ++indent_;
+    }
+    void close(std::string const &tail = "}") {
+        --indent_;
+        line(tail);
+    }
+    std::string fresh(std::string const &prefix) { return prefix + std::to_string(counter_++); }
+    std::string str() const { return os_.str(); }
+
+  private:
+    std::ostringstream os_;
+    int indent_ = 0;
+    int counter_ = 0;
+};
+
+/// Types and literals of one precision.
+struct fp_helper {
+    precision fp;
+    char const *real() const { return fp == precision::f32 ? "float" : "double"; }
+    char const *cplx() const { return fp == precision::f32 ? "float2" : "double2"; }
+    std::string lit(double v) const {
+        char buf[64];
+        if (fp == precision::f32) {
+            std::snprintf(buf, sizeof(buf), "%af", static_cast<double>(static_cast<float>(v)));
+        } else {
+            std::snprintf(buf, sizeof(buf), "%a", v);
+        }
+        return buf;
+    }
+    /// Literal pair (re, im) of exp(2 pi i k / n) rounded once from quad precision to T.
+    std::pair<std::string, std::string> root(long k, long n) const {
+        if (fp == precision::f32) {
+            auto w = gen::root_of_unity_f(k, n);
+            return {lit(w.real()), lit(w.imag())};
+        }
+        auto w = gen::root_of_unity(k, n);
+        return {lit(w.real()), lit(w.imag())};
+    }
+    /// Literal of cos / sin (2 pi k / n).
+    std::string cos_lit(long k, long n) const { return root(k, n).first; }
+    std::string sin_lit(long k, long n) const { return root(k, n).second; }
+    std::string zero2() const { return std::string("mk(") + lit(0.0) + ", " + lit(0.0) + ")"; }
+};
+
+long modn(long a, long n) {
+    long r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+/// Straight-line DFT code generator working on named complex SSA values.
+class dft_gen {
+  public:
+    dft_gen(writer &w, fp_helper f, int dir) : w_(w), f_(f), dir_(dir) {}
+
+    std::string let(std::string const &expr) {
+        auto n = w_.fresh("r");
+        w_.line(std::string("const ") + f_.cplx() + " " + n + " = " + expr + ";");
+        return n;
+    }
+    std::string add(std::string const &a, std::string const &b) {
+        return let("c_add(" + a + ", " + b + ")");
+    }
+    std::string sub(std::string const &a, std::string const &b) {
+        return let("c_sub(" + a + ", " + b + ")");
+    }
+
+    /// x * exp(2 pi i k / n), with trivial and diagonal constants simplified.
+    std::string mul_root(std::string const &x, long k, long n) {
+        long r = modn(k, n);
+        if (r == 0) {
+            return x;
+        }
+        if (2 * r == n) {
+            return let("c_neg(" + x + ")");
+        }
+        if (4 * r == n) {
+            return let("c_ipos(" + x + ")");
+        }
+        if (4 * r == 3 * n) {
+            return let("c_ineg(" + x + ")");
+        }
+        auto [c, s] = f_.root(r, n);
+        if ((8 * r) % n == 0) {
+            // odd multiple of pi/4: w = s0 (a + i b), a, b = +-1
+            long oct = 8 * r / n; // 1, 3, 5, 7
+            int a = (oct == 1 || oct == 7) ? 1 : -1;
+            int b = (oct == 1 || oct == 3) ? 1 : -1;
+            std::string h = f_.lit(0.70710678118654752440);
+            // (x.x a - x.y b, x.x b + x.y a) * s0
+            std::string re = std::string(a > 0 ? "" : "-") + x + ".x" + (b > 0 ? " - " : " + ") +
+                             x + ".y";
+            std::string im = std::string(b > 0 ? "" : "-") + x + ".x" + (a > 0 ? " + " : " - ") +
+                             x + ".y";
+            return let("c_scale(mk(" + re + ", " + im + "), " + h + ")");
+        }
+        return let("c_mulri(" + x + ", " + c + ", " + s + ")");
+    }
+
+    /// y[k] = sum_i x[i] exp(2 pi i dir i k / R), natural order.
+    std::vector<std::string> dft(std::vector<std::string> const &x) {
+        int const R = static_cast<int>(x.size());
+        if (R == 1) {
+            return x;
+        }
+        if (R == 2) {
+            return {add(x[0], x[1]), sub(x[0], x[1])};
+        }
+        if (R == 4) {
+            return radix4(x);
+        }
+        if (is_prime(static_cast<unsigned>(R))) {
+            return prime(x);
+        }
+        int A = R % 4 == 0 ? 4 : trial_division(R).front();
+        if (A == R) {
+            A = trial_division(R).front();
+        }
+        return split(x, A, R / A);
+    }
+
+  private:
+    std::vector<std::string> radix4(std::vector<std::string> const &x) {
+        auto t0 = add(x[0], x[2]);
+        auto t1 = sub(x[0], x[2]);
+        auto t2 = add(x[1], x[3]);
+        auto d = sub(x[1], x[3]);
+        // w4^dir * d: forward multiplies by -i, backward by +i
+        auto t3 = let(std::string(dir_ < 0 ? "c_ineg(" : "c_ipos(") + d + ")");
+        return {add(t0, t2), add(t1, t3), sub(t0, t2), sub(t1, t3)};
+    }
+
+    /// Odd prime R with the symmetric-pair formula:
+    /// y_k = x_0 + sum_j a_j cos(2 pi jk/R) + dir i sum_j b_j sin(2 pi jk/R), a_j = x_j + x_{R-j},
+    /// b_j = x_j - x_{R-j}; y_{R-k} flips the sign of the sine sum.
+    std::vector<std::string> prime(std::vector<std::string> const &x) {
+        int const R = static_cast<int>(x.size());
+        int const h = (R - 1) / 2;
+        std::vector<std::string> a(h + 1), b(h + 1);
+        for (int j = 1; j <= h; ++j) {
+            a[j] = add(x[j], x[R - j]);
+            b[j] = sub(x[j], x[R - j]);
+        }
+        std::vector<std::string> y(R);
+        {
+            std::string re = x[0] + ".x", im = x[0] + ".y";
+            for (int j = 1; j <= h; ++j) {
+                re += " + " + a[j] + ".x";
+                im += " + " + a[j] + ".y";
+            }
+            y[0] = let("mk(" + re + ", " + im + ")");
+        }
+        for (int k = 1; k <= h; ++k) {
+            std::string cr = x[0] + ".x", ci = x[0] + ".y", sr, si;
+            for (int j = 1; j <= h; ++j) {
+                long jk = static_cast<long>(j) * k;
+                std::string c = f_.cos_lit(jk, R);
+                std::string s = f_.sin_lit(jk, R);
+                cr += " + " + a[j] + ".x * " + c;
+                ci += " + " + a[j] + ".y * " + c;
+                sr += std::string(j > 1 ? " + " : "") + b[j] + ".x * " + s;
+                si += std::string(j > 1 ? " + " : "") + b[j] + ".y * " + s;
+            }
+            auto C = let("mk(" + cr + ", " + ci + ")");
+            auto S = let("mk(" + sr + ", " + si + ")");
+            // y_k = C + dir i S, y_{R-k} = C - dir i S; i S = (-S.y, S.x)
+            if (dir_ < 0) {
+                y[k] = let("mk(" + C + ".x + " + S + ".y, " + C + ".y - " + S + ".x)");
+                y[R - k] = let("mk(" + C + ".x - " + S + ".y, " + C + ".y + " + S + ".x)");
+            } else {
+                y[k] = let("mk(" + C + ".x - " + S + ".y, " + C + ".y + " + S + ".x)");
+                y[R - k] = let("mk(" + C + ".x + " + S + ".y, " + C + ".y - " + S + ".x)");
+            }
+        }
+        return y;
+    }
+
+    /// R = A B: A-point DFTs over x[B i1 + i2], twiddle w_R^{dir i2 k1}, B-point DFTs,
+    /// output index k1 + A k2.
+    std::vector<std::string> split(std::vector<std::string> const &x, int A, int B) {
+        int const R = A * B;
+        std::vector<std::vector<std::string>> z(B);
+        for (int i2 = 0; i2 < B; ++i2) {
+            std::vector<std::string> in(A);
+            for (int i1 = 0; i1 < A; ++i1) {
+                in[i1] = x[B * i1 + i2];
+            }
+            z[i2] = dft(in);
+            for (int k1 = 0; k1 < A; ++k1) {
+                z[i2][k1] = mul_root(z[i2][k1], static_cast<long>(dir_) * i2 * k1, R);
+            }
+        }
+        std::vector<std::string> y(R);
+        for (int k1 = 0; k1 < A; ++k1) {
+            std::vector<std::string> in(B);
+            for (int i2 = 0; i2 < B; ++i2) {
+                in[i2] = z[i2][k1];
+            }
+            auto out = dft(in);
+            for (int k2 = 0; k2 < B; ++k2) {
+                y[k1 + A * k2] = out[k2];
+            }
+        }
+        return y;
+    }
+
+    writer &w_;
+    fp_helper f_;
+    int dir_;
+};
+
+std::string u(std::size_t v) { return std::to_string(v) + "u"; }
+std::string ul(std::size_t v) { return std::to_string(v) + "ul"; }
+
+/// Emits one kernel.
+class kernel_gen {
+  public:
+    kernel_gen(fft1d_configuration const &c, std::string name)
+        : c_(c), f_{c.fp}, name_(std::move(name)) {
+        nfft_ = c.N_fft();
+        L_ = c.factorization.size();
+        real_ = c.type != transform_type::c2c;
+        half_ = real_ && c.N % 2 == 0;
+        pairs_ = real_ && c.N % 2 == 1;
+        use_lds_ = L_ > 1 || real_;
+        MG_ = (c.M + c.Mb - 1) / c.Mb;
+        if (c.lds_pad) {
+            pad_shift_ = 0;
+            while ((std::size_t(1) << pad_shift_) < c.lds_pad) {
+                ++pad_shift_;
+            }
+        }
+        np_ = (c.lds_pad ? (nfft_ - 1) + (nfft_ - 1) / c.lds_pad : nfft_ - 1) + 1;
+    }
+
+    std::string generate() {
+        emit_tables();
+        emit_kernel();
+        return w_.str();
+    }
+
+  private:
+    // ---- helpers -------------------------------------------------------------------------
+    std::string T() const { return f_.real(); }
+    std::string T2() const { return f_.cplx(); }
+
+    /// LDS slot of FFT index n (expression) within this work-item's column.
+    std::string lds(std::string const &n) const {
+        std::string p = c_.lds_pad ? "(" + n + " + ((" + n + ") >> " +
+                                         std::to_string(pad_shift_) + "))"
+                                   : "(" + n + ")";
+        return "col[" + p + " * " + u(c_.Mb) + "]";
+    }
+
+    std::string in_type() const {
+        return c_.type == transform_type::r2c ? T() : T2();
+    }
+    std::string out_type() const {
+        return c_.type == transform_type::c2r ? T() : T2();
+    }
+
+    /// Global load of input element at offset expression `off` (elements from `in`).
+    std::string gload(std::string const &off) const {
+        if (c_.load_function) {
+            return std::string(c_.load_function) + "((" + in_type() + "*)in, " + off + ")";
+        }
+        return "in[" + off + "]";
+    }
+    /// Global store of `val` at offset `off` (elements from `out`).
+    std::string gstore(std::string const &off, std::string const &val) const {
+        if (c_.store_function) {
+            return std::string(c_.store_function) + "(out, " + off + ", " + val + ");";
+        }
+        return "out[" + off + "] = " + val + ";";
+    }
+
+    std::string tw_name(std::size_t s) const { return name_ + "_tw" + std::to_string(s); }
+    std::string twr_name() const { return name_ + "_twr"; }
+
+    void emit_table(std::string const &name, std::vector<std::pair<long, long>> const &roots,
+                    bool times_i) {
+        w_.line("static __constant__ " + T2() + " " + name + "[" + std::to_string(roots.size()) +
+                "] = {");
+        std::string row;
+        for (std::size_t i = 0; i < roots.size(); ++i) {
+            auto [re, im] = f_.root(roots[i].first, roots[i].second);
+            if (times_i) {
+                // i * (re + i im) = -im + i re
+                std::string nim = im[0] == '-' ? im.substr(1) : "-" + im;
+                row += "{" + nim + ", " + re + "}, ";
+            } else {
+                row += "{" + re + ", " + im + "}, ";
+            }
+            if ((i + 1) % 4 == 0 || i + 1 == roots.size()) {
+                w_.line("  " + row);
+                row.clear();
+            }
+        }
+        w_.line("};");
+    }
+
+    void emit_tables() {
+        std::size_t p = c_.factorization[0];
+        for (std::size_t s = 1; s < L_; ++s) {
+            std::size_t R = c_.factorization[s];
+            std::vector<std::pair<long, long>> roots;
+            for (std::size_t jp = 0; jp < p; ++jp) {
+                for (std::size_t i = 0; i < R; ++i) {
+                    roots.push_back({static_cast<long>(c_.direction) * long(i) * long(jp),
+                                     long(p * R)});
+                }
+            }
+            emit_table(tw_name(s), roots, false);
+            p *= R;
+        }
+        if (half_) {
+            // i * w_N^{dir q}, q = 0..H/2 (reference: sbfft_gen.cpp:192,238)
+            std::vector<std::pair<long, long>> roots;
+            for (std::size_t q = 0; q <= nfft_ / 2; ++q) {
+                roots.push_back({static_cast<long>(c_.direction) * long(q), long(c_.N)});
+            }
+            emit_table(twr_name(), roots, true);
+        }
+    }
+
+    // ---- kernel ----------------------------------------------------------------------------
+    void emit_kernel() {
+        std::size_t const wg = c_.work_group_size();
+        w_.line("extern \"C\" __global__ void __launch_bounds__(" + std::to_string(wg) + ") " +
+                name_ + "(const " + in_type() + "* in, " + out_type() +
+                "* out, unsigned long K) {");
+        w_.open("");
+        if (use_lds_) {
+            w_.line("__shared__ " + T2() + " lds_buf[" + std::to_string(c_.Mb * c_.Kb * np_) +
+                    "];");
+        }
+        w_.line("const unsigned lid = threadIdx.x;");
+        w_.line("const unsigned ml = lid % " + u(c_.Mb) + ";");
+        w_.line("const unsigned t = (lid / " + u(c_.Mb) + ") % " + u(c_.Nt) + ";");
+        w_.line("const unsigned kl = lid / " + u(c_.Mb * c_.Nt) + ";");
+        if (pairs_) {
+            w_.line("const unsigned long kcols = (K + 1ul) / 2ul;");
+        } else {
+            w_.line("const unsigned long kcols = K;");
+        }
+        w_.line("const unsigned long ntiles = " + ul(MG_) + " * ((kcols + " + ul(c_.Kb - 1) +
+                ") / " + ul(c_.Kb) + ");");
+        if (use_lds_) {
+            w_.line(T2() + "* col = lds_buf + ml + " + u(c_.Mb * np_) + " * kl;");
+        }
+        w_.open("for (unsigned long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)");
+        if (MG_ == 1) {
+            w_.line("const unsigned long m = ml;");
+            w_.line("const unsigned long kc = tile * " + ul(c_.Kb) + " + kl;");
+        } else {
+            w_.line("const unsigned long m = (tile % " + ul(MG_) + ") * " + ul(c_.Mb) + " + ml;");
+            w_.line("const unsigned long kc = (tile / " + ul(MG_) + ") * " + ul(c_.Kb) +
+                    " + kl;");
+        }
+        std::string act = "kc < kcols";
+        if (MG_ * c_.Mb != c_.M) {
+            act = "m < " + ul(c_.M) + " && " + act;
+        }
+        w_.line("const bool active = " + act + ";");
+        if (pairs_) {
+            w_.line("const bool active_b = active && 2ul * kc + 1ul < K;");
+            w_.line("const unsigned long ib = m + 2ul * kc * " + ul(c_.istride[2]) + ";");
+            w_.line("const unsigned long ob = m + 2ul * kc * " + ul(c_.ostride[2]) + ";");
+        } else {
+            w_.line("const unsigned long ib = m + kc * " + ul(c_.istride[2]) + ";");
+            w_.line("const unsigned long ob = m + kc * " + ul(c_.ostride[2]) + ";");
+        }
+
+        if (c_.type == transform_type::c2r) {
+            emit_c2r_pre();
+            w_.line("__syncthreads();");
+        }
+        for (std::size_t s = 0; s < L_; ++s) {
+            emit_stage(s);
+        }
+        if (c_.type == transform_type::r2c) {
+            emit_r2c_post();
+        }
+        if (use_lds_) {
+            w_.line("__syncthreads();");
+        }
+        w_.close();
+        w_.close();
+        w_.line("}");
+    }
+
+    /// Offset (elements) of FFT index n in the input column along mode 1.
+    std::string ioff(std::string const &n, std::size_t extra_k = 0) const {
+        std::string k = extra_k ? " + " + ul(c_.istride[2] * extra_k) : "";
+        return "ib" + k + " + (unsigned long)(" + n + ") * " + ul(c_.istride[1]);
+    }
+    std::string ooff(std::string const &n, std::size_t extra_k = 0) const {
+        std::string k = extra_k ? " + " + ul(c_.ostride[2] * extra_k) : "";
+        return "ob" + k + " + (unsigned long)(" + n + ") * " + ul(c_.ostride[1]);
+    }
+
+    bool vector_real_in() const {
+        return half_ && c_.type == transform_type::r2c && !c_.load_function &&
+               c_.istride[1] == 1 && c_.istride[2] % 2 == 0 && c_.M == 1;
+    }
+    bool vector_real_out() const {
+        return half_ && c_.type == transform_type::c2r && !c_.store_function &&
+               c_.ostride[1] == 1 && c_.ostride[2] % 2 == 0 && c_.M == 1;
+    }
+
+    /// Load z[n] of the complex FFT input from global memory into variable v.
+    void emit_global_input(std::string const &v, std::string const &n) {
+        if (c_.type == transform_type::c2c) {
+            w_.line(v + " = " + gload(ioff(n)) + ";");
+        } else if (half_) {
+            if (vector_real_in()) {
+                w_.line(v + " = ((const " + T2() + "*)(in + ib))[" + n + "];");
+            } else {
+                w_.line(v + " = mk(" + gload(ioff("2u * (" + n + ")")) + ", " +
+                        gload(ioff("2u * (" + n + ") + 1u")) + ");");
+            }
+        } else { // pairs
+            w_.line(v + ".x = " + gload(ioff(n)) + ";");
+            w_.line("if (active_b) " + v + ".y = " + gload(ioff(n, 1)) + ";");
+        }
+    }
+
+    /// Store FFT output y at index n to global memory.
+    void emit_global_output(std::string const &y, std::string const &n) {
+        if (c_.type == transform_type::c2c) {
+            w_.line(gstore(ooff(n), y));
+        } else if (half_) {
+            if (vector_real_out()) {
+                w_.line("((" + T2() + "*)(out + ob))[" + n + "] = " + y + ";");
+            } else {
+                w_.line(gstore(ooff("2u * (" + n + ")"), y + ".x"));
+                w_.line(gstore(ooff("2u * (" + n + ") + 1u"), y + ".y"));
+            }
+        } else { // pairs
+            w_.line(gstore(ooff(n), y + ".x"));
+            w_.line("if (active_b) " + gstore(ooff(n, 1), y + ".y"));
+        }
+    }
+
+    void emit_stage(std::size_t s) {
+        std::size_t const R = c_.factorization[s];
+        std::size_t p = 1;
+        for (std::size_t i = 0; i < s; ++i) {
+            p *= c_.factorization[i];
+        }
+        std::size_t const NR = nfft_ / R;
+        std::size_t const B = (NR + c_.Nt - 1) / c_.Nt;
+        bool const src_global = s == 0 && c_.type != transform_type::c2r;
+        bool const dst_global = s + 1 == L_ && c_.type != transform_type::r2c;
+        bool const mid_barrier = !src_global && !dst_global;
+
+        w_.line("// stage " + std::to_string(s) + ": radix " + std::to_string(R) + ", p = " +
+                std::to_string(p));
+        std::vector<std::vector<std::string>> v(B);
+        std::vector<std::string> guard(B), jn(B);
+        for (std::size_t b = 0; b < B; ++b) {
+            jn[b] = "j" + std::to_string(s) + "_" + std::to_string(b);
+            w_.line("const unsigned " + jn[b] + " = t + " + u(b * c_.Nt) + ";");
+            if ((b + 1) * c_.Nt > NR) {
+                guard[b] = jn[b] + " < " + u(NR);
+            }
+            v[b].resize(R);
+            for (std::size_t i = 0; i < R; ++i) {
+                v[b][i] = "v" + std::to_string(s) + "_" + std::to_string(b) + "_" +
+                          std::to_string(i);
+                w_.line(T2() + " " + v[b][i] + " = " + f_.zero2() + ";");
+            }
+            std::string cond = guard[b];
+            if (src_global) {
+                cond = cond.empty() ? "active" : "active && " + cond;
+            }
+            if (!cond.empty()) {
+                w_.open("if (" + cond + ")");
+            }
+            for (std::size_t i = 0; i < R; ++i) {
+                std::string n = jn[b] + " + " + u(i * NR);
+                if (src_global) {
+                    emit_global_input(v[b][i], n);
+                } else {
+                    w_.line(v[b][i] + " = " + lds(n) + ";");
+                }
+            }
+            if (s > 0) {
+                std::string jp = "(" + jn[b] + " % " + u(p) + ") * " + u(R);
+                for (std::size_t i = 1; i < R; ++i) {
+                    w_.line(v[b][i] + " = c_mul(" + v[b][i] + ", " + tw_name(s) + "[" + jp +
+                            " + " + u(i) + "]);");
+                }
+            }
+            if (!cond.empty()) {
+                w_.close();
+            }
+        }
+        if (mid_barrier) {
+            w_.line("__syncthreads();");
+        }
+        for (std::size_t b = 0; b < B; ++b) {
+            std::string cond = guard[b];
+            if (dst_global) {
+                cond = cond.empty() ? "active" : "active && " + cond;
+            }
+            // The butterfly is computed unconditionally when only the store is predicated;
+            // scoping keeps the SSA temporaries local.
+            if (!guard[b].empty()) {
+                w_.open("if (" + guard[b] + ")");
+            } else {
+                w_.open("");
+            }
+            dft_gen d(w_, f_, c_.direction);
+            auto y = d.dft(v[b]);
+            std::string const &j = jn[b];
+            if (dst_global) {
+                if (guard[b].empty()) {
+                    w_.open("if (active)");
+                } else {
+                    w_.open("if (active)");
+                }
+                for (std::size_t i = 0; i < R; ++i) {
+                    emit_global_output(y[i], j + " + " + u(i * NR));
+                }
+                w_.close();
+            } else {
+                std::string base = p == 1 ? j + " * " + u(R)
+                                          : "(" + j + " / " + u(p) + ") * " + u(p * R) + " + " +
+                                                j + " % " + u(p);
+                for (std::size_t i = 0; i < R; ++i) {
+                    w_.line(lds(base + " + " + u(i * p)) + " = " + y[i] + ";");
+                }
+            }
+            w_.close();
+        }
+        if (!dst_global) {
+            w_.line("__syncthreads();");
+        }
+    }
+
+    /// c2r: Hermitian input -> complex FFT input z in LDS (reference: sbfft_gen.cpp:221-247
+    /// for even N, 318-351 for odd N; the imaginary part of the DC bin is ignored).
+    void emit_c2r_pre() {
+        w_.line("// c2r pre-pass");
+        std::size_t const H = nfft_;
+        std::size_t const Q = half_ ? H / 2 + 1 : (c_.N - 1) / 2 + 1;
+        std::size_t const B = (Q + c_.Nt - 1) / c_.Nt;
+        for (std::size_t b = 0; b < B; ++b) {
+            std::string q = "q" + std::to_string(b);
+            w_.line("const unsigned " + q + " = t + " + u(b * c_.Nt) + ";");
+            w_.open("if (active && " + q + " < " + u(Q) + ")");
+            if (half_) {
+                w_.line(T2() + " x1 = " + gload(ioff(q)) + ";");
+                w_.line("if (" + q + " == 0u) x1.y = " + f_.lit(0.0) + ";");
+                w_.line(T2() + " x2 = " + gload(ioff(u(H) + " - " + q)) + ";");
+                w_.line("x2.y = -x2.y;");
+                w_.line("const " + T2() + " a = c_add(x1, x2);");
+                w_.line("const " + T2() + " bt = c_mul(c_sub(x1, x2), " + twr_name() + "[" + q +
+                        "]);");
+                w_.line(lds(q + " % " + u(H)) + " = c_add(a, bt);");
+                w_.line("if (" + q + " != 0u && 2u * " + q + " != " + u(H) + ") " +
+                        lds(u(H) + " - " + q) + " = mk(a.x - bt.x, bt.y - a.y);");
+            } else {
+                w_.line(T2() + " ai = " + gload(ioff(q)) + ";");
+                w_.line(T2() + " bi = " + f_.zero2() + ";");
+                w_.line("if (active_b) bi = " + gload(ioff(q, 1)) + ";");
+                w_.line("if (" + q + " == 0u) { ai.y = " + f_.lit(0.0) + "; bi.y = " +
+                        f_.lit(0.0) + "; }");
+                w_.line("const " + T2() + " ib2 = c_ipos(bi);");
+                w_.line(lds(q) + " = c_add(ai, ib2);");
+                w_.line("if (" + q + " != 0u) { const " + T2() + " d = c_sub(ai, ib2); " +
+                        lds(u(c_.N) + " - " + q) + " = mk(d.x, -d.y); }");
+            }
+            w_.close();
+        }
+    }
+
+    /// r2c: complex FFT output in LDS -> Hermitian half spectrum (reference: sbfft_gen.cpp:180-199
+    /// for even N, 275-291 for odd N).
+    void emit_r2c_post() {
+        w_.line("// r2c post-pass");
+        std::size_t const H = nfft_;
+        std::size_t const Q = half_ ? H / 2 + 1 : (c_.N - 1) / 2 + 1;
+        std::size_t const B = (Q + c_.Nt - 1) / c_.Nt;
+        std::string const half = f_.lit(0.5);
+        for (std::size_t b = 0; b < B; ++b) {
+            std::string q = "q" + std::to_string(b);
+            w_.line("const unsigned " + q + " = t + " + u(b * c_.Nt) + ";");
+            w_.open("if (active && " + q + " < " + u(Q) + ")");
+            if (half_) {
+                w_.line("const " + T2() + " y1 = " + lds(q + " % " + u(H)) + ";");
+                w_.line(T2() + " y2 = " + lds("(" + u(H) + " - " + q + ") % " + u(H)) + ";");
+                w_.line("y2.y = -y2.y;");
+                w_.line("const " + T2() + " a = c_scale(c_add(y2, y1), " + half + ");");
+                w_.line("const " + T2() + " bt = c_mul(c_scale(c_sub(y2, y1), " + half + "), " +
+                        twr_name() + "[" + q + "]);");
+                w_.line(gstore(ooff(q), "c_add(a, bt)"));
+                w_.line("if (2u * " + q + " != " + u(H) + ") " +
+                        gstore(ooff(u(H) + " - " + q), "mk(a.x - bt.x, bt.y - a.y)"));
+            } else {
+                w_.line("const " + T2() + " y1 = " + lds(q) + ";");
+                w_.line(T2() + " y2 = " + lds("(" + u(c_.N) + " - " + q + ") % " + u(c_.N)) +
+                        ";");
+                w_.line("y2.y = -y2.y;");
+                w_.line(gstore(ooff(q), "c_scale(c_add(y2, y1), " + half + ")"));
+                w_.line("if (active_b) { const " + T2() + " d = c_scale(c_sub(y2, y1), " + half +
+                        "); " + gstore(ooff(q, 1), "mk(-d.y, d.x)") + " }");
+            }
+            w_.close();
+        }
+    }
+
+    fft1d_configuration const &c_;
+    fp_helper f_;
+    std::string name_;
+    writer w_;
+    std::size_t nfft_ = 0, L_ = 0, MG_ = 1, np_ = 0;
+    unsigned pad_shift_ = 0;
+    bool real_ = false, half_ = false, pairs_ = false, use_lds_ = false;
+};
+
+} // namespace
+
+/// Complex helpers shared by all kernels of a source; overloaded for float2 and double2.
+std::string hip_prelude() {
+    std::ostringstream os;
+    os << "#ifndef BBFFT_HIP_PRELUDE\n#define BBFFT_HIP_PRELUDE\n";
+    for (char const *T : {"float", "double"}) {
+        std::string t = T, t2 = t + "2";
+        os << "static __device__ __forceinline__ " << t2 << " mk(" << t << " a, " << t
+           << " b) { " << t2 << " r; r.x = a; r.y = b; return r; }\n";
+        os << "static __device__ __forceinline__ " << t2 << " c_add(" << t2 << " a, " << t2
+           << " b) { return mk(a.x + b.x, a.y + b.y); }\n";
+        os << "static __device__ __forceinline__ " << t2 << " c_sub(" << t2 << " a, " << t2
+           << " b) { return mk(a.x - b.x, a.y - b.y); }\n";
+        os << "static __device__ __forceinline__ " << t2 << " c_mul(" << t2 << " a, " << t2
+           << " w) { return mk(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x); }\n";
+        os << "static __device__ __forceinline__ " << t2 << " c_mulri(" << t2 << " a, " << t
+           << " c, " << t << " s) { return mk(a.x * c - a.y * s, a.x * s + a.y * c); }\n";
+        os << "static __device__ __forceinline__ " << t2 << " c_scale(" << t2 << " a, " << t
+           << " s) { return mk(a.x * s, a.y * s); }\n";
+        os << "static __device__ __forceinline__ " << t2 << " c_ipos(" << t2
+           << " a) { return mk(-a.y, a.x); }\n";
+        os << "static __device__ __forceinline__ " << t2 << " c_ineg(" << t2
+           << " a) { return mk(a.y, -a.x); }\n";
+        os << "static __device__ __forceinline__ " << t2 << " c_neg(" << t2
+           << " a) { return mk(-a.x, -a.y); }\n";
+    }
+    os << "#endif\n";
+    return os.str();
+}
+
+void generate_fft1d(std::ostream &os, fft1d_configuration const &cfg, std::string_view name) {
+    if (cfg.factorization.empty()) {
+        throw bad_configuration("empty factorization");
+    }
+    std::string n = name.empty() ? cfg.identifier() : std::string(name);
+    os << hip_prelude();
+    kernel_gen g(cfg, n);
+    os << g.generate();
+}
+
+} // namespace bbfft

@@ -1,0 +1,76 @@
+// Internal plan classes of the HIP back end (shared by make_plan and the C ABI).
+#ifndef BBFFT_AMD_HIP_PLAN_IMPL_HPP
+#define BBFFT_AMD_HIP_PLAN_IMPL_HPP
+
+#include "../plan/decompose.hpp"
+#include "bbfft/configuration.hpp"
+#include "bbfft/detail/generator_impl.hpp"
+#include "bbfft/detail/plan_impl.hpp"
+#include "bbfft/jit_cache.hpp"
+#include "bbfft/shared_handle.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace bbfft::hip_detail {
+
+/// One kernel launch of a plan (for introspection, profiling and tests).
+struct launch_info {
+    std::string kernel_name;
+    unsigned grid;
+    unsigned block;
+    std::size_t lds_bytes;
+};
+
+class plan_base : public detail::plan_unmanaged_event_impl<hipEvent_t> {
+  public:
+    virtual std::vector<launch_info> launches() const = 0;
+};
+
+/// A single 1-D double-batched kernel.
+class fft1d_plan final : public plan_base {
+  public:
+    fft1d_plan(configuration const &cfg, hipStream_t stream, int device, jit_cache *cache);
+    void execute(void const *in, void *out, hipEvent_t signal, std::uint32_t nwait,
+                 hipEvent_t *wait) override;
+    /// Launch on the plan's stream without event handling.
+    void launch(void const *in, void *out);
+    std::vector<launch_info> launches() const override;
+    fft1d_configuration const &kernel_config() const { return kc_; }
+
+  private:
+    hipStream_t stream_;
+    int device_;
+    fft1d_configuration kc_;
+    std::string name_;
+    shared_handle<module_handle_t> mod_;
+    hipFunction_t fn_ = nullptr;
+    unsigned long K_ = 0;
+    unsigned grid_ = 0, block_ = 0;
+    std::size_t in_align_ = 1, out_align_ = 1;
+};
+
+/// 2-D/3-D transform as a chain of 1-D passes on one stream.
+class nd_plan final : public plan_base {
+  public:
+    nd_plan(pass_layout const &layout, hipStream_t stream, int device, jit_cache *cache);
+    ~nd_plan();
+    void execute(void const *in, void *out, hipEvent_t signal, std::uint32_t nwait,
+                 hipEvent_t *wait) override;
+    std::vector<launch_info> launches() const override;
+
+  private:
+    hipStream_t stream_;
+    std::vector<std::unique_ptr<fft1d_plan>> passes_;
+    void *tmp_ = nullptr;
+};
+
+std::shared_ptr<plan_base> create(configuration const &cfg, hipStream_t stream,
+                                  jit_cache *cache);
+
+} // namespace bbfft::hip_detail
+
+#endif // BBFFT_AMD_HIP_PLAN_IMPL_HPP

@@ -1,0 +1,190 @@
+// HIP plans: module lookup/JIT, launch geometry and event handling.
+//
+// Reference behaviour: src/common/algorithm.hpp:20-30 (dimension check and dispatch),
+// src/common/algorithm/small_batch_fft.hpp:42-139 and factor2_slm_fft.hpp:92-203 (cache lookup,
+// generate on miss, in-place guard, bind (in, out, K) and launch), nd_fft.hpp:114-176 (chained
+// passes through a scratch buffer). One kernel family replaces the two reference families, the
+// kernel arguments are (in, out, K) and the launch grid is linearised over (M tile, K tile).
+#include "bbfft/hip/make_plan.hpp"
+#include "../plan/decompose.hpp"
+#include "bbfft/bad_configuration.hpp"
+#include "bbfft/detail/generator_impl.hpp"
+#include "bbfft/hip/device.hpp"
+#include "bbfft/hip/error.hpp"
+#include "bbfft/hip/online_compiler.hpp"
+#include "hip_plan_impl.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <memory>
+#include <sstream>
+
+namespace bbfft {
+
+namespace {
+
+void release_module(module_handle_t m) { (void)hipModuleUnload(reinterpret_cast<hipModule_t>(m)); }
+
+/// User callback source as it is placed in front of the generated kernel. OpenCL-C callbacks
+/// (the reference's only language) get address-space qualifiers mapped away and are compiled as
+/// host+device functions.
+std::string callback_source(user_module const &um) {
+    if (!um) {
+        return {};
+    }
+    std::string src(um.data, um.length);
+    if (um.language == kernel_language::opencl_c) {
+        return "#define global\n#define __global\n#define kernel\n#define __kernel\n"
+               "typedef unsigned int uint;\ntypedef unsigned long ulong;\n"
+               "typedef unsigned short ushort;\ntypedef unsigned char uchar;\n"
+               "#pragma clang force_cuda_host_device begin\n" +
+               src +
+               "\n#pragma clang force_cuda_host_device end\n"
+               "#undef global\n#undef __global\n#undef kernel\n#undef __kernel\n";
+    }
+    return src + "\n";
+}
+
+} // namespace
+
+namespace hip_detail {
+
+fft1d_plan::fft1d_plan(configuration const &cfg, hipStream_t stream, int device,
+                       jit_cache *cache)
+    : stream_(stream), device_(device) {
+    kc_ = configure_fft1d(cfg, hip::get_device_info(device));
+    name_ = kc_.identifier();
+    K_ = cfg.shape[2];
+    jit_cache_key key{name_, hip::get_device_id(device)};
+    if (cache) {
+        mod_ = cache->get(key);
+    }
+    if (!mod_) {
+        std::ostringstream src;
+        src << callback_source(cfg.callbacks);
+        generate_fft1d(src, kc_);
+        hipModule_t m = hip::build_kernel_bundle(src.str(), device);
+        mod_ = shared_handle<module_handle_t>(reinterpret_cast<module_handle_t>(m),
+                                              &release_module);
+        if (cache) {
+            cache->store(key, mod_);
+        }
+    }
+    fn_ = hip::create_kernel(reinterpret_cast<hipModule_t>(mod_.get()), name_);
+    std::size_t const mg = (kc_.M + kc_.Mb - 1) / kc_.Mb;
+    std::size_t const kcols = kc_.K_columns(K_);
+    std::size_t const ntiles = mg * ((kcols + kc_.Kb - 1) / kc_.Kb);
+    grid_ = static_cast<unsigned>(std::min<std::size_t>(ntiles, std::size_t(1) << 24));
+    block_ = static_cast<unsigned>(kc_.work_group_size());
+    std::size_t const rb = static_cast<std::size_t>(kc_.fp);
+    // Real data read or written as (x[2n], x[2n+1]) pairs needs complex alignment.
+    in_align_ = kc_.type == transform_type::r2c ? rb : 2 * rb;
+    out_align_ = kc_.type == transform_type::c2r ? rb : 2 * rb;
+    if (kc_.N % 2 == 0 && kc_.M == 1 && kc_.type == transform_type::r2c &&
+        !kc_.load_function && kc_.istride[1] == 1 && kc_.istride[2] % 2 == 0) {
+        in_align_ = 2 * rb;
+    }
+    if (kc_.N % 2 == 0 && kc_.M == 1 && kc_.type == transform_type::c2r &&
+        !kc_.store_function && kc_.ostride[1] == 1 && kc_.ostride[2] % 2 == 0) {
+        out_align_ = 2 * rb;
+    }
+}
+
+void fft1d_plan::launch(void const *in, void *out) {
+    if (in == out && kc_.inplace_unsupported) {
+        throw bad_configuration("The plan does not support in-place transform on the current "
+                                "device. Please use the out-of-place transform.");
+    }
+    if (reinterpret_cast<std::uintptr_t>(in) % in_align_ ||
+        reinterpret_cast<std::uintptr_t>(out) % out_align_) {
+        throw bad_configuration("Input/output pointers are not aligned to the element size");
+    }
+    void const *a0 = in;
+    void *a1 = out;
+    unsigned long k = K_;
+    void *args[] = {&a0, &a1, &k};
+    BBFFT_HIP_CHECK(
+        hipModuleLaunchKernel(fn_, grid_, 1, 1, block_, 1, 1, 0, stream_, args, nullptr));
+}
+
+void fft1d_plan::execute(void const *in, void *out, hipEvent_t signal, std::uint32_t nwait,
+                         hipEvent_t *wait) {
+    for (std::uint32_t i = 0; i < nwait; ++i) {
+        BBFFT_HIP_CHECK(hipStreamWaitEvent(stream_, wait[i], 0));
+    }
+    launch(in, out);
+    if (signal) {
+        BBFFT_HIP_CHECK(hipEventRecord(signal, stream_));
+    }
+}
+
+std::vector<launch_info> fft1d_plan::launches() const {
+    return {launch_info{name_, grid_, block_, kc_.lds_bytes()}};
+}
+
+nd_plan::nd_plan(pass_layout const &layout, hipStream_t stream, int device, jit_cache *cache)
+    : stream_(stream) {
+    for (auto const &p : layout.passes) {
+        passes_.push_back(std::make_unique<fft1d_plan>(p, stream, device, cache));
+    }
+    if (layout.tmp_bytes) {
+        BBFFT_HIP_CHECK(hipMalloc(&tmp_, layout.tmp_bytes));
+    }
+}
+
+nd_plan::~nd_plan() {
+    if (tmp_) {
+        (void)hipFree(tmp_);
+    }
+}
+
+void nd_plan::execute(void const *in, void *out, hipEvent_t signal, std::uint32_t nwait,
+                      hipEvent_t *wait) {
+    for (std::uint32_t i = 0; i < nwait; ++i) {
+        BBFFT_HIP_CHECK(hipStreamWaitEvent(stream_, wait[i], 0));
+    }
+    // All passes run on one stream, so stream order replaces the reference's internal events.
+    void *tmp = tmp_ ? tmp_ : out;
+    std::size_t const D = passes_.size();
+    passes_[0]->launch(in, tmp);
+    for (std::size_t d = 1; d + 1 < D; ++d) {
+        passes_[d]->launch(tmp, tmp);
+    }
+    passes_[D - 1]->launch(tmp, out);
+    if (signal) {
+        BBFFT_HIP_CHECK(hipEventRecord(signal, stream_));
+    }
+}
+
+std::vector<launch_info> nd_plan::launches() const {
+    std::vector<launch_info> all;
+    for (auto const &p : passes_) {
+        auto l = p->launches();
+        all.insert(all.end(), l.begin(), l.end());
+    }
+    return all;
+}
+
+std::shared_ptr<plan_base> create(configuration const &cfg, hipStream_t stream,
+                                  jit_cache *cache) {
+    if (cfg.dim < 1 || cfg.dim > max_fft_dim) {
+        throw bad_configuration("Unsupported FFT dimension: " + std::to_string(cfg.dim));
+    }
+    int device = 0;
+    BBFFT_HIP_CHECK(hipGetDevice(&device));
+    auto layout = decompose(cfg);
+    if (layout.passes.size() == 1) {
+        return std::make_shared<fft1d_plan>(layout.passes[0], stream, device, cache);
+    }
+    return std::make_shared<nd_plan>(layout, stream, device, cache);
+}
+
+} // namespace hip_detail
+
+auto make_plan(configuration const &cfg, hipStream_t stream, jit_cache *cache) -> hip_plan {
+    return hip_plan(hip_detail::create(cfg, stream, cache));
+}
+
+} // namespace bbfft

@@ -1,0 +1,93 @@
+#include "decompose.hpp"
+#include "bbfft/bad_configuration.hpp"
+
+#include <string>
+#include <utility>
+
+namespace bbfft {
+
+namespace {
+bool same_strides(std::array<std::size_t, max_tensor_dim> const &a,
+                  std::array<std::size_t, max_tensor_dim> const &b, unsigned dim) {
+    for (unsigned d = 0; d < dim + 2; ++d) {
+        if (a[d] != b[d]) {
+            return false;
+        }
+    }
+    return true;
+}
+} // namespace
+
+pass_layout decompose(configuration const &cfg) {
+    if (cfg.dim < 1 || cfg.dim > max_fft_dim) {
+        throw bad_configuration("Unsupported FFT dimension: " + std::to_string(cfg.dim));
+    }
+    pass_layout out;
+    if (cfg.dim == 1) {
+        configuration c = cfg;
+        c.shape = {cfg.shape[0], cfg.shape[1], cfg.shape[2], 0, 0};
+        out.passes.push_back(c);
+        return out;
+    }
+    if (cfg.callbacks) {
+        throw bad_configuration("User modules are unsupported for FFT dimension > 1.");
+    }
+    auto is_default = [&](bool inplace) {
+        return same_strides(cfg.istride, default_istride(cfg.dim, cfg.shape, cfg.type, inplace),
+                            cfg.dim) &&
+               same_strides(cfg.ostride, default_ostride(cfg.dim, cfg.shape, cfg.type, inplace),
+                            cfg.dim);
+    };
+    bool const inplace_layout = is_default(true);
+    if (!inplace_layout && !is_default(false)) {
+        throw bad_configuration("Only default tensor layouts are supported for the nd_fft.");
+    }
+    bool const real = cfg.type != transform_type::c2c;
+    unsigned const D = cfg.dim;
+    // Extent of mode d on the complex side and on the real side (in-place r2c pads mode 0).
+    auto n_complex = [&](unsigned d) { return d == 0 && real ? cfg.shape[1] / 2 + 1 : cfg.shape[d + 1]; };
+    auto n_real = [&](unsigned d) {
+        return d == 0 && real && inplace_layout ? 2 * (cfg.shape[1] / 2 + 1) : cfg.shape[d + 1];
+    };
+    std::size_t prod_complex = 1;
+    for (unsigned d = 0; d < D; ++d) {
+        prod_complex *= n_complex(d);
+    }
+    std::vector<configuration> p1d(D);
+    std::size_t M = cfg.shape[0];
+    std::size_t K = prod_complex * cfg.shape[D + 1];
+    for (unsigned d = 0; d < D; ++d) {
+        std::size_t const Nd = cfg.shape[d + 1];
+        K /= n_complex(d);
+        configuration c = {};
+        c.dim = 1;
+        c.shape = {M, Nd, K, 0, 0};
+        c.fp = cfg.fp;
+        c.dir = cfg.dir;
+        c.type = d == 0 ? cfg.type : transform_type::c2c;
+        c.istride = {1, M, M * n_real(d), 0, 0};
+        c.ostride = {1, M, M * n_complex(d), 0, 0};
+        p1d[d] = c;
+        M *= n_complex(d);
+    }
+    if (cfg.type == transform_type::c2r) {
+        for (unsigned d = 0; d < D; ++d) {
+            auto c = p1d[D - 1 - d];
+            std::swap(c.istride, c.ostride);
+            out.passes.push_back(c);
+        }
+    } else {
+        out.passes = std::move(p1d);
+    }
+    std::size_t const rb = static_cast<std::size_t>(cfg.fp);
+    std::size_t const ib = cfg.type == transform_type::r2c ? rb : 2 * rb;
+    std::size_t const ob = cfg.type == transform_type::c2r ? rb : 2 * rb;
+    std::size_t const isize = cfg.istride[D + 1] * cfg.shape[D + 1] * ib;
+    std::size_t const osize = cfg.ostride[D + 1] * cfg.shape[D + 1] * ob;
+    if (isize > osize) {
+        out.tmp_bytes = isize;
+    }
+    return out;
+}
+
+} // namespace bbfft

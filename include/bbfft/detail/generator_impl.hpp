@@ -1,0 +1,80 @@
+// Kernel-level configuration of the gfx950 generator and its entry points.
+// Plays the role of the reference's include/bbfft/detail/generator_impl.hpp:26-107, where two
+// kernel families (small_batch_configuration, factor2_slm_configuration) exist; on gfx950 one
+// family covers both: a single radix stage owned by one work-item is the small-batch case and
+// several stages exchanged through LDS are the multi-factor case.
+#ifndef BBFFT_DETAIL_GENERATOR_IMPL_HPP
+#define BBFFT_DETAIL_GENERATOR_IMPL_HPP
+
+#include "bbfft/configuration.hpp"
+#include "bbfft/device_info.hpp"
+#include "bbfft/export.hpp"
+
+#include <array>
+#include <cstddef>
+#include <iosfwd>
+#include <string>
+#include <string_view>
+#include <vector>
+
+namespace bbfft {
+
+/**
+ * @brief Tiling and algorithm of one generated 1-D double-batched FFT kernel.
+ *
+ * A work-group of Mb * Nt * Kb work-items owns Mb consecutive M-columns times Kb consecutive
+ * K-columns (K-column pairs for real transforms of odd length). Nt work-items cooperate on one
+ * transform: the complex FFT of length N_fft runs as len(factorization) Stockham stages, stage s
+ * with radix factorization[s] in registers; stages exchange data through padded LDS.
+ */
+struct BBFFT_EXPORT fft1d_configuration {
+    int direction;                       ///< -1 forward, +1 backward
+    std::size_t M;                       ///< M (left batch)
+    std::size_t Mb;                      ///< M tile
+    std::size_t N;                       ///< transform length (real length for r2c/c2r)
+    std::vector<int> factorization;      ///< stage radices of the complex FFT, first stage first
+    std::size_t Nt;                      ///< work-items per transform
+    std::size_t Kb;                      ///< K tile
+    precision fp;                        ///< precision
+    transform_type type;                 ///< c2c / r2c / c2r
+    std::array<std::size_t, 3u> istride; ///< {1, s_N, s_K} of the input
+    std::array<std::size_t, 3u> ostride; ///< {1, s_N, s_K} of the output
+    std::size_t lds_pad;                 ///< one padding slot per lds_pad slots (0 = none)
+    bool inplace_unsupported;            ///< in == out would race across work-groups
+    char const *load_function;           ///< user load callback or nullptr
+    char const *store_function;          ///< user store callback or nullptr
+
+    /// Kernel symbol name; also the jit_cache key.
+    std::string identifier() const;
+    /// Length of the complex FFT (N/2 for real even N, else N).
+    std::size_t N_fft() const;
+    /// True for real transforms of odd length (two K-columns share one complex FFT).
+    bool pairs_k() const;
+    /// Number of K-columns (or K-column pairs) for a given K.
+    std::size_t K_columns(std::size_t K) const;
+    std::size_t work_group_size() const { return Mb * Nt * Kb; }
+    /// Bytes of LDS used by one work-group.
+    std::size_t lds_bytes() const;
+};
+
+/// Optional overrides of the tiling heuristics (0 / empty = choose automatically).
+struct BBFFT_EXPORT fft1d_tuning {
+    std::vector<int> factorization;
+    std::size_t Nt = 0;
+    std::size_t Mb = 0;
+    std::size_t Kb = 0;
+    std::size_t lds_pad = 0;
+    bool no_pad = false;
+};
+
+/// Choose a kernel configuration for a 1-D configuration (dim == 1).
+BBFFT_EXPORT fft1d_configuration configure_fft1d(configuration const &cfg,
+                                                 device_info const &info,
+                                                 fft1d_tuning const &tuning = {});
+/// Emit HIP source (no includes; hiprtc provides the HIP device builtins) for one kernel.
+BBFFT_EXPORT void generate_fft1d(std::ostream &os, fft1d_configuration const &cfg,
+                                 std::string_view name = {});
+
+} // namespace bbfft
+
+#endif // BBFFT_DETAIL_GENERATOR_IMPL_HPP

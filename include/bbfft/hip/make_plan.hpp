@@ -1,0 +1,30 @@
+// make_plan for HIP streams.
+// Mirrors the reference's include/bbfft/{cl,ze,sycl}/make_plan.hpp; like the Level Zero back end
+// the plan uses caller-owned events (hipEvent_t).
+#ifndef BBFFT_HIP_MAKE_PLAN_HPP
+#define BBFFT_HIP_MAKE_PLAN_HPP
+
+#include "bbfft/configuration.hpp"
+#include "bbfft/export.hpp"
+#include "bbfft/jit_cache.hpp"
+#include "bbfft/plan.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+namespace bbfft {
+
+using hip_plan = plan_unmanaged_event<hipEvent_t>;
+
+/**
+ * @brief Create a plan for cfg on the current HIP device; kernels run on `stream`.
+ *
+ * Kernels come from `cache` when present there, else they are generated and compiled with
+ * hiprtc (and stored into `cache`). Throws bad_configuration for unsupported configurations and
+ * bbfft::hip::error for runtime/compiler failures.
+ */
+BBFFT_EXPORT auto make_plan(configuration const &cfg, hipStream_t stream,
+                            jit_cache *cache = nullptr) -> hip_plan;
+
+} // namespace bbfft
+
+#endif // BBFFT_HIP_MAKE_PLAN_HPP

@@ -1,0 +1,146 @@
+/*
+ * C ABI of the MI355X (gfx950) double-batched FFT library.
+ *
+ * Every entry point returns an int status (BBFFT_SUCCESS = 0) and never lets a C++ exception
+ * cross the boundary; bbfft_last_error() returns the message of the last failure on the calling
+ * thread. Streams and events are hipStream_t / hipEvent_t passed as opaque pointers, and tensors
+ * are device pointers from hipMalloc (or any allocator returning device-accessible memory).
+ *
+ * Each declaration cites the reference interface it replaces (paths relative to the reference
+ * repository intel/double-batched-fft-library).
+ */
+#ifndef BBFFT_HIP_H
+#define BBFFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BBFFT_EXPORT_C __attribute__((visibility("default")))
+
+enum bbfft_status {
+    BBFFT_SUCCESS = 0,
+    BBFFT_ERROR_BAD_CONFIGURATION = 1, /* bbfft::bad_configuration */
+    BBFFT_ERROR_RUNTIME = 2,           /* HIP runtime / hiprtc failure (bbfft::hip::error) */
+    BBFFT_ERROR_INVALID_ARGUMENT = 3,  /* null handle, short buffer, index out of range */
+    BBFFT_ERROR_PARSE = 4,             /* malformed descriptor (std::runtime_error in C++) */
+    BBFFT_ERROR_INTERNAL = 5
+};
+
+typedef struct bbfft_plan_s *bbfft_plan_t;
+typedef struct bbfft_cache_s *bbfft_cache_t;
+typedef void *bbfft_stream_t; /* hipStream_t */
+typedef void *bbfft_event_t;  /* hipEvent_t */
+
+/*
+ * Mirror of bbfft::configuration (include/bbfft/configuration.hpp:151-192) and
+ * bbfft::user_module (include/bbfft/user_module.hpp:16-37).
+ * precision: 4 (f32) or 8 (f64); direction: -1 forward, +1 backward;
+ * type: 0 c2c, 1 r2c, 2 c2r; callback_language: 0 OpenCL-C style, 1 HIP.
+ */
+typedef struct bbfft_configuration {
+    unsigned dim;
+    size_t shape[5];
+    int precision;
+    int direction;
+    int type;
+    size_t istride[5];
+    size_t ostride[5];
+    const char *callback_source;
+    size_t callback_length;
+    const char *load_function;
+    const char *store_function;
+    int callback_language;
+} bbfft_configuration;
+
+/* Tiling of one generated kernel (include/bbfft/detail/generator_impl.hpp: fft1d_configuration;
+ * replaces the reference's small_batch_configuration / factor2_slm_configuration,
+ * include/bbfft/detail/generator_impl.hpp:26-107 of the reference). */
+typedef struct bbfft_kernel_config {
+    size_t M, Mb, N, Nt, Kb, lds_pad, lds_bytes, work_group_size;
+    int factorization[8];
+    int num_factors;
+    int inplace_unsupported;
+} bbfft_kernel_config;
+
+/* Message of the last failed call on this thread ("" if none). */
+BBFFT_EXPORT_C const char *bbfft_last_error(void);
+/* Library version string. */
+BBFFT_EXPORT_C const char *bbfft_version(void);
+
+/* parse_fft_descriptor (src/base/parser.cpp:59-211, include/bbfft/parser.hpp:24). */
+BBFFT_EXPORT_C int bbfft_parse_fft_descriptor(const char *desc, bbfft_configuration *cfg);
+/* configuration::to_string / operator<< (src/base/configuration.cpp:63-160). */
+BBFFT_EXPORT_C int bbfft_configuration_to_string(const bbfft_configuration *cfg, char *buf,
+                                                 size_t len);
+/* default_istride / default_ostride (src/base/configuration.cpp:24-56). */
+BBFFT_EXPORT_C int bbfft_default_istride(unsigned dim, const size_t shape[5], int type,
+                                         int inplace, size_t stride[5]);
+BBFFT_EXPORT_C int bbfft_default_ostride(unsigned dim, const size_t shape[5], int type,
+                                         int inplace, size_t stride[5]);
+/* parse_device_info + device_info::to_string (src/base/parser.cpp:213-270,
+ * src/base/device_info.cpp:59-95): parse `desc` and print it canonically into buf. */
+BBFFT_EXPORT_C int bbfft_device_info_normalize(const char *desc, char *buf, size_t len);
+/* Integer helpers of the factorisation logic (src/base/math.cpp:8-71,
+ * src/base/prime_factorization.cpp:13-77): factor(n, index) writes `index` factors. */
+BBFFT_EXPORT_C int bbfft_factor(unsigned n, unsigned index, unsigned *factors);
+BBFFT_EXPORT_C unsigned bbfft_iroot(unsigned radicand, unsigned index);
+BBFFT_EXPORT_C int bbfft_is_prime(unsigned n);
+
+/* Kernel configuration and HIP source of the 1-D pass `pass` of cfg for a device-info string
+ * (configure_* + generate_*, include/bbfft/detail/generator_impl.hpp:51-107 of the reference).
+ * No GPU is needed. bbfft_generate_source writes at most len bytes and sets *needed. */
+BBFFT_EXPORT_C int bbfft_num_passes(const bbfft_configuration *cfg, unsigned *num);
+BBFFT_EXPORT_C int bbfft_kernel_configuration(const bbfft_configuration *cfg, unsigned pass,
+                                              const char *device_info,
+                                              bbfft_kernel_config *out, char *name,
+                                              size_t name_len);
+BBFFT_EXPORT_C int bbfft_generate_source(const bbfft_configuration *cfg, unsigned pass,
+                                         const char *device_info, char *buf, size_t len,
+                                         size_t *needed);
+
+/* Module caches: jit_cache_all (include/bbfft/jit_cache_all.hpp:20-37) layered over an
+ * aot_cache (include/bbfft/aot_cache.hpp:21-48). bbfft_cache_load_aot registers an archive
+ * written by bbfft_aot_generate (tools/aot/main.cpp:22-60 + aot_cache.register_module). */
+BBFFT_EXPORT_C int bbfft_cache_create(bbfft_cache_t *cache);
+BBFFT_EXPORT_C int bbfft_cache_load_aot(bbfft_cache_t cache, const char *path, int device,
+                                        size_t *num_kernels);
+BBFFT_EXPORT_C int bbfft_cache_size(bbfft_cache_t cache, size_t *num_modules);
+BBFFT_EXPORT_C void bbfft_cache_destroy(bbfft_cache_t cache);
+
+/* make_plan(cfg, queue, cache) (include/bbfft/ze/make_plan.hpp:29-31,
+ * include/bbfft/sycl/make_plan.hpp:27-41): plan on the current HIP device for `stream`. */
+BBFFT_EXPORT_C int bbfft_plan_create(const bbfft_configuration *cfg, bbfft_stream_t stream,
+                                     bbfft_cache_t cache, bbfft_plan_t *plan);
+BBFFT_EXPORT_C int bbfft_plan_create_from_descriptor(const char *desc, bbfft_stream_t stream,
+                                                     bbfft_cache_t cache, bbfft_plan_t *plan);
+/* plan_unmanaged_event::execute (include/bbfft/plan.hpp:165-183): waits for `wait`, launches,
+ * records `signal` (both optional); in == out is the in-place transform. Asynchronous. */
+BBFFT_EXPORT_C int bbfft_plan_execute(bbfft_plan_t plan, const void *in, void *out,
+                                      bbfft_event_t signal, uint32_t num_wait,
+                                      const bbfft_event_t *wait);
+/* Introspection of the kernels a plan launches. */
+BBFFT_EXPORT_C int bbfft_plan_num_launches(bbfft_plan_t plan, unsigned *num);
+BBFFT_EXPORT_C int bbfft_plan_launch_info(bbfft_plan_t plan, unsigned index, char *name,
+                                          size_t name_len, unsigned *grid, unsigned *block,
+                                          size_t *lds_bytes);
+BBFFT_EXPORT_C void bbfft_plan_destroy(bbfft_plan_t plan);
+
+/* Device info of a HIP device as a device-info string (src/cl/device.cpp:12-40 pattern). */
+BBFFT_EXPORT_C int bbfft_device_info_query(int device, char *buf, size_t len);
+
+/* bbfft-aot-generate (tools/aot/main.cpp:22-60): generate every kernel the descriptors need for
+ * device_info, compile them with hiprtc for `arch` (e.g. "gfx950") using `jobs` threads and
+ * write an archive to out_path. No GPU is needed. */
+BBFFT_EXPORT_C int bbfft_aot_generate(const char *const *descriptors, size_t num_descriptors,
+                                      const char *device_info, const char *arch,
+                                      const char *out_path, int jobs, size_t *num_kernels);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BBFFT_HIP_H */

@@ -1,0 +1,117 @@
+// Host emulation of the small HIP subset used by the generated kernels, for CPU tests of the
+// generator: each work-item is a ucontext fiber, __syncthreads() yields to a scheduler that runs
+// all fibers of a work-group to the next barrier, work-groups run one after another.
+// Test infrastructure only; the library never runs kernels this way.
+#ifndef BBFFT_TEST_HIP_EMU_HPP
+#define BBFFT_TEST_HIP_EMU_HPP
+
+#include <cstddef>
+#include <cstdio>
+#include <cstdlib>
+#include <sys/mman.h>
+#include <ucontext.h>
+#include <vector>
+
+struct float2 {
+    float x, y;
+};
+struct double2 {
+    double x, y;
+};
+struct emu_dim3 {
+    unsigned x = 0, y = 0, z = 0;
+};
+
+#define __global__
+#define __device__
+#define __forceinline__ inline
+#define __constant__
+#define __shared__ static
+#define __launch_bounds__(x)
+
+namespace emu {
+inline emu_dim3 threadIdx_, blockIdx_, gridDim_, blockDim_;
+inline ucontext_t sched_ctx;
+inline std::vector<ucontext_t> fiber_ctx;
+inline std::vector<int> fiber_state; // 0 runnable, 1 at barrier, 2 done
+inline unsigned current = 0;
+inline std::size_t barrier_count = 0;
+} // namespace emu
+
+#define threadIdx (emu::threadIdx_)
+#define blockIdx (emu::blockIdx_)
+#define gridDim (emu::gridDim_)
+#define blockDim (emu::blockDim_)
+
+inline void __syncthreads() {
+    unsigned me = emu::current;
+    emu::fiber_state[me] = 1;
+    swapcontext(&emu::fiber_ctx[me], &emu::sched_ctx);
+}
+
+namespace emu {
+
+/// Runs body() for `block` work-items of each of `grid` work-groups. Returns 0 on success,
+/// 1 if work-items of one group disagree on the number of barriers (divergent barrier).
+template <typename Body> int launch(unsigned grid, unsigned block, Body body) {
+    static std::size_t const stack_size = 512 * 1024;
+    static Body *body_ptr;
+    body_ptr = &body;
+    fiber_ctx.assign(block, ucontext_t{});
+    fiber_state.assign(block, 0);
+    std::vector<char *> stacks(block);
+    for (unsigned i = 0; i < block; ++i) {
+        stacks[i] = static_cast<char *>(mmap(nullptr, stack_size, PROT_READ | PROT_WRITE,
+                                             MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0));
+    }
+    gridDim_.x = grid;
+    blockDim_.x = block;
+    int rc = 0;
+    for (unsigned b = 0; b < grid && rc == 0; ++b) {
+        blockIdx_.x = b;
+        for (unsigned i = 0; i < block; ++i) {
+            getcontext(&fiber_ctx[i]);
+            fiber_ctx[i].uc_stack.ss_sp = stacks[i];
+            fiber_ctx[i].uc_stack.ss_size = stack_size;
+            fiber_ctx[i].uc_link = &sched_ctx;
+            makecontext(&fiber_ctx[i], (void (*)()) + [] {
+                (*body_ptr)();
+                fiber_state[current] = 2;
+            }, 0);
+            fiber_state[i] = 0;
+        }
+        for (;;) {
+            unsigned done = 0, at_barrier = 0;
+            for (unsigned i = 0; i < block; ++i) {
+                if (fiber_state[i] == 0) {
+                    current = i;
+                    threadIdx_.x = i;
+                    swapcontext(&sched_ctx, &fiber_ctx[i]);
+                }
+            }
+            for (unsigned i = 0; i < block; ++i) {
+                done += fiber_state[i] == 2;
+                at_barrier += fiber_state[i] == 1;
+            }
+            if (done == block) {
+                break;
+            }
+            if (done != 0 && at_barrier != 0) {
+                rc = 1; // some work-items finished while others wait at a barrier
+                break;
+            }
+            ++barrier_count;
+            for (unsigned i = 0; i < block; ++i) {
+                fiber_state[i] = 0;
+            }
+        }
+    }
+    for (unsigned i = 0; i < block; ++i) {
+        munmap(stacks[i], stack_size);
+    }
+    return rc;
+}
+
+} // namespace emu
+
+#endif

@@ -1,0 +1,89 @@
+"""Run generated kernels on the CPU through the fiber emulator (tests/emu/hip_emu.hpp).
+
+Test infrastructure: exercises the generator's indexing, LDS exchange, barriers and real
+pre/post passes without a GPU. Compiled kernels are cached under tests/emu/_build/.
+"""
+import ctypes
+import hashlib
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_BUILD = os.path.join(_HERE, "_build")
+
+
+def compile_kernel(source, kernel_name, in_type, out_type, callback_source=""):
+    """Compile `source` with the emulation shim; returns a ctypes launcher."""
+    os.makedirs(_BUILD, exist_ok=True)
+    tu = ('#include "hip_emu.hpp"\n' + callback_source + "\n" + source + "\n"
+          'extern "C" int emu_launch(const void* in, void* out, unsigned long K, unsigned grid, '
+          "unsigned block) {\n"
+          f"  return emu::launch(grid, block, [&] {{ {kernel_name}(({in_type}*)in, "
+          f"({out_type}*)out, K); }});\n}}\n")
+    h = hashlib.sha1(tu.encode()).hexdigest()[:16]
+    so = os.path.join(_BUILD, f"k_{h}.so")
+    if not os.path.exists(so):
+        src = os.path.join(_BUILD, f"k_{h}.cpp")
+        with open(src, "w") as f:
+            f.write(tu)
+        tmp = so + f".tmp{os.getpid()}"
+        subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-w", "-I", _HERE, src,
+                        "-o", tmp], check=True)
+        os.replace(tmp, so)
+    lib = ctypes.CDLL(so)
+    fn = lib.emu_launch
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulong, ctypes.c_uint, ctypes.c_uint]
+    fn.restype = ctypes.c_int
+    return fn
+
+
+def run(bbfft, cfg, x, out_size, grid_cap=None, device_info=None, callback_source=""):
+    """Execute the 1-D configuration `cfg` (bbfft_amd.configuration) on host arrays.
+
+    x: numpy array (complex64/128 or float32/64) holding the input tensor;
+    returns a numpy array of out_size elements (complex or real per the transform type)."""
+    info = device_info or bbfft.GFX950_INFO
+    kc = bbfft.kernel_configuration(cfg, 0, info)
+    src = bbfft.generate_source(cfg, 0, info)
+    f32 = int(cfg.fp) == 4
+    real_t, cplx_t = ("float", "float2") if f32 else ("double", "double2")
+    npc, npr = (np.complex64, np.float32) if f32 else (np.complex128, np.float64)
+    t = int(cfg.type)
+    in_t = real_t if t == 1 else cplx_t
+    out_t = real_t if t == 2 else cplx_t
+    fn = compile_kernel(src, kc["name"], in_t, out_t, callback_source)
+    M, N, K = cfg.shape[0], cfg.shape[1], cfg.shape[2]
+    pairs = t != 0 and N % 2 == 1
+    kcols = (K + 1) // 2 if pairs else K
+    mg = -(-M // kc["Mb"])
+    ntiles = mg * (-(-kcols // kc["Kb"]))
+    grid = ntiles if grid_cap is None else min(ntiles, grid_cap)
+    xin = np.ascontiguousarray(x, dtype=npr if t == 1 else npc)
+    out = np.zeros(out_size, dtype=npr if t == 2 else npc)
+    rc = fn(xin.ctypes.data, out.ctypes.data, K, grid, kc["work_group_size"])
+    if rc != 0:
+        raise RuntimeError("divergent barrier in emulated kernel")
+    return out
+
+
+def run_inplace(bbfft, cfg, buf, grid_cap=None, device_info=None):
+    """In-place execution on one host buffer (numpy array of the input element type)."""
+    info = device_info or bbfft.GFX950_INFO
+    kc = bbfft.kernel_configuration(cfg, 0, info)
+    src = bbfft.generate_source(cfg, 0, info)
+    f32 = int(cfg.fp) == 4
+    real_t, cplx_t = ("float", "float2") if f32 else ("double", "double2")
+    t = int(cfg.type)
+    fn = compile_kernel(src, kc["name"], real_t if t == 1 else cplx_t,
+                        real_t if t == 2 else cplx_t)
+    M, N, K = cfg.shape[0], cfg.shape[1], cfg.shape[2]
+    pairs = t != 0 and N % 2 == 1
+    kcols = (K + 1) // 2 if pairs else K
+    ntiles = -(-M // kc["Mb"]) * (-(-kcols // kc["Kb"]))
+    grid = ntiles if grid_cap is None else min(ntiles, grid_cap)
+    rc = fn(buf.ctypes.data, buf.ctypes.data, K, grid, kc["work_group_size"])
+    if rc != 0:
+        raise RuntimeError("divergent barrier in emulated kernel")
+    return buf
