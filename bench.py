@@ -1,0 +1,136 @@
+"""Benchmark of the double-batched 1-D FFT on one MI355X.
+
+Workload (headline): plain c2c forward, fp32, M=8 x N=256 x K=32768 (descriptor
+"scfo8.256*32768"), out-of-place, inputs uniform[-1,1) resident in HBM. A step is one
+plan.execute over the whole tensor. Also reported: the sweep N in {16..512, 60, 105, 243, 384}
+for fp32 and fp64 (same M, K), a device-to-device copy bandwidth, and the CPU baseline (the
+oracle's C port of the reference algorithm on a bounded sample, host cores).
+
+Accounting (reference benchmark/adapter.hpp:51-52): flops = 5 N log2 N M K, bytes = 2 M N K
+sizeof(complex). Timing: HIP events on the plan's stream around the timed steps.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "double-batched-fft-library_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "GFLOP/s (5N·log2N) + %HBM peak; MFMA-split vs plain fp32 at equal FP64-err"
+
+
+def time_plan(bb, torch, desc, steps, warmup, cache):
+    cfg = bb.parse_fft_descriptor(desc)
+    M, N, K = cfg.shape[:3]
+    fp = int(cfg.fp)
+    rdt = torch.float32 if fp == 4 else torch.float64
+    g = torch.Generator(device="cuda")
+    g.manual_seed(N)
+    x = torch.view_as_complex(torch.rand((K, N, M, 2), generator=g, device="cuda", dtype=rdt)
+                              * 2 - 1).contiguous()
+    y = torch.empty_like(x)
+    plan = bb.make_plan(cfg, cache=cache)
+    for _ in range(warmup):
+        plan.execute(x, y)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        plan.execute(x, y)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    flops = 5.0 * N * math.log2(N) * M * K
+    nbytes = 2.0 * M * N * K * 2 * fp
+    launch = plan.launches()[0]
+    del x, y
+    return {"desc": desc, "ms": ms, "gflops": flops / ms / 1e6, "GBps": nbytes / ms / 1e6,
+            "frac": nbytes / ms / 1e6 / HBM_PEAK_GBPS, "bytes": nbytes, "kernel": launch}
+
+
+def copy_bandwidth(torch, nbytes=1 << 30, reps=10):
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e.record()
+    torch.cuda.synchronize()
+    return 2.0 * nbytes * reps / (s.elapsed_time(e) / 1e3) / 1e9
+
+
+def cpu_baseline(budget_s=15.0):
+    """Oracle port of the reference algorithm on host cores, bounded sample of the workload."""
+    try:
+        import oracle.cpu_port as port
+    except Exception:  # noqa: BLE001 - the baseline is optional context
+        return None
+    return port.benchmark(M=8, N=256, budget_s=budget_s)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--desc", default="scfo8.256*32768")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the double-batched FFT "
+              "shards K across GPUs with no exchange, so run it once per GPU.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import bbfft_amd as bb
+    cache = bb.default_cache()
+    head = time_plan(bb, torch, a.desc, a.steps, a.warmup, cache)
+    sweep = []
+    if not a.no_sweep:
+        for p in "sd":
+            for n in (16, 32, 64, 128, 256, 512, 60, 105, 243, 384):
+                r = time_plan(bb, torch, f"{p}cfo8.{n}*32768", max(5, a.steps // 2), 2, cache)
+                sweep.append({"desc": r["desc"], "ms": round(r["ms"], 5),
+                              "GFLOPs": round(r["gflops"], 1), "GBps": round(r["GBps"], 1),
+                              "frac": round(r["frac"], 4)})
+                print(json.dumps(sweep[-1]), file=sys.stderr, flush=True)
+    copy_gbps = copy_bandwidth(torch)
+    cpu = None if a.no_cpu else cpu_baseline()
+    cfg = bb.parse_fft_descriptor(a.desc)
+    line = {
+        "metric": METRIC,
+        "value": round(head["gflops"], 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(head["ms"], 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if int(cfg.fp) == 4 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"c2c forward {a.desc}: M={cfg.shape[0]} x N={cfg.shape[1]} "
+                               f"x K={cfg.shape[2]}, out-of-place, uniform[-1,1) complex",
+                   "descriptor": a.desc, "HBM_GBps": round(head["GBps"], 1),
+                   "hbm_copy_GBps": round(copy_gbps, 1), "kernel": head["kernel"]["kernel"],
+                   "sweep": sweep},
+        "roofline": {"bound": "hbm", "achieved": round(head["GBps"], 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(head["frac"], 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -11,6 +11,7 @@ PyTorch (when present) is only used for device pointers and the current stream.
 """
 import ctypes
 import enum
+import os
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
@@ -175,7 +176,23 @@ def normalize_device_info(desc: str) -> str:
     return buf.value.decode()
 
 
-GFX950_INFO = "{1024, {64}, 65536, gpu}"
+# What hipDeviceProp_t reports for MI355X (maxThreadsPerBlock, warpSize, sharedMemPerBlock).
+GFX950_INFO = "{1024, {64}, 163840, gpu}"
+
+
+def root_of_unity(k, n, precision_=8) -> complex:
+    """exp(2 pi i k / n) correctly rounded to the given precision (as emitted in kernels)."""
+    re, im = ctypes.c_double(), ctypes.c_double()
+    _check(_capi.load().bbfft_root_of_unity(k, n, int(precision_), ctypes.byref(re),
+                                            ctypes.byref(im)))
+    return complex(re.value, im.value)
+
+
+def factor(n, index):
+    """factor(n, index) of the library's tiling logic."""
+    out = (ctypes.c_uint * max(1, index))()
+    _check(_capi.load().bbfft_factor(n, index, out))
+    return list(out)[:index]
 
 
 def num_passes(cfg: configuration) -> int:
@@ -209,14 +226,32 @@ def generate_source(cfg: configuration, pass_index=0, device_info=GFX950_INFO) -
     return buf.value.decode()
 
 
-def aot_generate(descriptors: List[str], out_path: str, device_info=GFX950_INFO,
-                 arch="gfx950", jobs=8) -> int:
+def compile_kernel(cfg: configuration, pass_index=0, device_info=GFX950_INFO,
+                   arch="gfx950") -> int:
+    """Compile the kernel of one pass (callbacks included) with hiprtc; returns its size."""
+    n = ctypes.c_size_t()
+    _check(_capi.load().bbfft_compile_kernel(ctypes.byref(cfg._to_c()), pass_index,
+                                             device_info.encode(), arch.encode(),
+                                             ctypes.byref(n)))
+    return n.value
+
+
+def aot_generate_shard(descriptors: List[str], out_path: str, shard=0, num_shards=1,
+                       device_info=GFX950_INFO, arch="gfx950") -> int:
+    """Compile shard `shard` of the kernels of `descriptors` into one archive (no GPU)."""
     arr = (ctypes.c_char_p * len(descriptors))(*[d.encode() for d in descriptors])
     n = ctypes.c_size_t()
     _check(_capi.load().bbfft_aot_generate(arr, len(descriptors), device_info.encode(),
-                                           arch.encode(), out_path.encode(), jobs,
+                                           arch.encode(), out_path.encode(), shard, num_shards,
                                            ctypes.byref(n)))
     return n.value
+
+
+def aot_generate(descriptors: List[str], out_dir: str, device_info=GFX950_INFO, arch="gfx950",
+                 jobs=8) -> List[str]:
+    """bbfft-aot-generate: archives out_dir/kernels_<i>.bin built by `jobs` processes."""
+    from .aot import generate
+    return generate(descriptors, out_dir, device_info=device_info, arch=arch, jobs=jobs)
 
 
 def device_info_query(device=0) -> str:
@@ -233,9 +268,18 @@ class Cache:
         _check(_capi.load().bbfft_cache_create(ctypes.byref(self._h)))
 
     def load_aot(self, path, device=0):
-        n = ctypes.c_size_t()
-        _check(_capi.load().bbfft_cache_load_aot(self._h, path.encode(), device, ctypes.byref(n)))
-        return n.value
+        """Register one archive file, or every kernels_*.bin archive of a directory."""
+        import glob
+        import os
+        paths = (sorted(glob.glob(os.path.join(path, "kernels_*.bin"))) if os.path.isdir(path)
+                 else [path])
+        total = 0
+        for p in paths:
+            n = ctypes.c_size_t()
+            _check(_capi.load().bbfft_cache_load_aot(self._h, p.encode(), device,
+                                                     ctypes.byref(n)))
+            total += n.value
+        return total
 
     def __len__(self):
         n = ctypes.c_size_t()
@@ -247,8 +291,9 @@ class Cache:
         return self._h
 
     def __del__(self):
-        if getattr(self, "_h", None) and self._h.value:
-            _capi.load().bbfft_cache_destroy(self._h)
+        lib = _capi._lib
+        if lib is not None and getattr(self, "_h", None) and self._h.value:
+            lib.bbfft_cache_destroy(self._h)
             self._h = ctypes.c_void_p()
 
 
@@ -303,9 +348,25 @@ class Plan:
         return out
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _capi.load().bbfft_plan_destroy(self._h)
+        lib = _capi._lib
+        if lib is not None and getattr(self, "_h", None):
+            lib.bbfft_plan_destroy(self._h)
             self._h = None
+
+
+DEFAULT_AOT_DIR = os.path.join(os.path.dirname(_capi.DEFAULT_LIB), "aot")
+_default_cache = None
+
+
+def default_cache(device=0) -> "Cache":
+    """Process-wide cache preloaded with the archives in lib/aot (if they exist)."""
+    global _default_cache
+    if _default_cache is None:
+        c = Cache()
+        if os.path.isdir(DEFAULT_AOT_DIR):
+            c.load_aot(DEFAULT_AOT_DIR, device)
+        _default_cache = c
+    return _default_cache
 
 
 def make_plan(cfg, stream=None, cache: Optional[Cache] = None) -> Plan:

@@ -15,7 +15,9 @@ EXPORTED = [
     "bbfft_last_error", "bbfft_version", "bbfft_parse_fft_descriptor",
     "bbfft_configuration_to_string", "bbfft_default_istride", "bbfft_default_ostride",
     "bbfft_device_info_normalize", "bbfft_factor", "bbfft_iroot", "bbfft_is_prime",
+    "bbfft_root_of_unity",
     "bbfft_num_passes", "bbfft_kernel_configuration", "bbfft_generate_source",
+    "bbfft_compile_kernel",
     "bbfft_cache_create", "bbfft_cache_load_aot", "bbfft_cache_size", "bbfft_cache_destroy",
     "bbfft_plan_create", "bbfft_plan_create_from_descriptor", "bbfft_plan_execute",
     "bbfft_plan_num_launches", "bbfft_plan_launch_info", "bbfft_plan_destroy",
@@ -91,11 +93,15 @@ def load():
         "bbfft_factor": (c_int, [c_uint, c_uint, P(c_uint)]),
         "bbfft_iroot": (c_uint, [c_uint, c_uint]),
         "bbfft_is_prime": (c_int, [c_uint]),
+        "bbfft_root_of_unity": (c_int, [ctypes.c_long, ctypes.c_long, c_int,
+                                        P(ctypes.c_double), P(ctypes.c_double)]),
         "bbfft_num_passes": (c_int, [P(CConfiguration), P(c_uint)]),
         "bbfft_kernel_configuration": (c_int, [P(CConfiguration), c_uint, c_char_p,
                                                P(CKernelConfig), c_char_p, c_size]),
         "bbfft_generate_source": (c_int, [P(CConfiguration), c_uint, c_char_p, c_char_p, c_size,
                                           P(c_size)]),
+        "bbfft_compile_kernel": (c_int, [P(CConfiguration), c_uint, c_char_p, c_char_p,
+                                         P(c_size)]),
         "bbfft_cache_create": (c_int, [P(c_void_p)]),
         "bbfft_cache_load_aot": (c_int, [c_void_p, c_char_p, c_int, P(c_size)]),
         "bbfft_cache_size": (c_int, [c_void_p, P(c_size)]),
@@ -110,8 +116,8 @@ def load():
                                            P(c_uint), P(c_size)]),
         "bbfft_plan_destroy": (None, [c_void_p]),
         "bbfft_device_info_query": (c_int, [c_int, c_char_p, c_size]),
-        "bbfft_aot_generate": (c_int, [P(c_char_p), c_size, c_char_p, c_char_p, c_char_p, c_int,
-                                       P(c_size)]),
+        "bbfft_aot_generate": (c_int, [P(c_char_p), c_size, c_char_p, c_char_p, c_char_p, c_uint,
+                                       c_uint, P(c_size)]),
     }
     for name, (res, args) in protos.items():
         fn = getattr(L, name)

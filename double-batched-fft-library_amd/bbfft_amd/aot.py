@@ -1,0 +1,72 @@
+"""bbfft-aot-generate: compile the kernels of FFT descriptors ahead of time (no GPU needed).
+
+    python -m bbfft_amd.aot -o OUT_DIR [-d DEVICE_INFO] [-a ARCH] [-j JOBS] DESC... | @FILE
+
+Reference: tools/aot/main.cpp:22-60 and tools/aot/args.cpp:17-102. hiprtc compiles one program
+at a time per process, so shards of the kernel list are built by separate processes, each
+writing OUT_DIR/kernels_<shard>.bin; a Cache loads the whole directory.
+"""
+import argparse
+import os
+import subprocess
+import sys
+
+_PKG_PARENT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def generate(descriptors, out_dir, device_info=None, arch="gfx950", jobs=8):
+    from . import GFX950_INFO
+    device_info = device_info or GFX950_INFO
+    os.makedirs(out_dir, exist_ok=True)
+    for f in os.listdir(out_dir):
+        if f.startswith("kernels_") and f.endswith(".bin"):
+            os.remove(os.path.join(out_dir, f))
+    listing = os.path.join(out_dir, "descriptors.txt")
+    with open(listing, "w") as f:
+        f.write("\n".join(descriptors) + "\n")
+    procs = []
+    env = dict(os.environ)
+    env["PYTHONPATH"] = _PKG_PARENT + os.pathsep + env.get("PYTHONPATH", "")
+    for shard in range(jobs):
+        out = os.path.join(out_dir, f"kernels_{shard}.bin")
+        cmd = [sys.executable, "-m", "bbfft_amd.aot", "--shard", str(shard), "--num-shards",
+               str(jobs), "-d", device_info, "-a", arch, "--out-file", out, "@" + listing]
+        procs.append(subprocess.Popen(cmd, env=env))
+    rc = [p.wait() for p in procs]
+    if any(rc):
+        raise RuntimeError(f"AOT generation failed (exit codes {rc})")
+    return sorted(os.path.join(out_dir, f) for f in os.listdir(out_dir)
+                  if f.startswith("kernels_"))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("-o", "--out-dir")
+    ap.add_argument("--out-file")
+    ap.add_argument("-d", "--device-info", default=None)
+    ap.add_argument("-a", "--arch", default="gfx950")
+    ap.add_argument("-j", "--jobs", type=int, default=8)
+    ap.add_argument("--shard", type=int, default=None)
+    ap.add_argument("--num-shards", type=int, default=1)
+    ap.add_argument("descriptors", nargs="+")
+    a = ap.parse_args(argv)
+    descs = []
+    for d in a.descriptors:
+        if d.startswith("@"):
+            with open(d[1:]) as f:
+                descs += [line.strip() for line in f if line.strip()]
+        else:
+            descs.append(d)
+    from . import GFX950_INFO, aot_generate_shard
+    info = a.device_info or GFX950_INFO
+    if a.shard is not None:
+        n = aot_generate_shard(descs, a.out_file, a.shard, a.num_shards, info, a.arch)
+        print(f"shard {a.shard}/{a.num_shards}: {n} kernels -> {a.out_file}", flush=True)
+        return 0
+    files = generate(descs, a.out_dir, info, a.arch, a.jobs)
+    print(f"{len(files)} archives in {a.out_dir}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -2,6 +2,7 @@
 #include "bbfft_hip.h"
 
 #include "../base/math.hpp"
+#include "../generator/roots.hpp"
 #include "../hip/aot_archive.hpp"
 #include "../hip/hip_plan_impl.hpp"
 #include "../plan/decompose.hpp"
@@ -17,7 +18,6 @@
 #include "bbfft/parser.hpp"
 
 #include <algorithm>
-#include <atomic>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -25,7 +25,6 @@
 #include <set>
 #include <sstream>
 #include <string>
-#include <thread>
 #include <vector>
 
 using namespace bbfft;
@@ -208,6 +207,21 @@ unsigned bbfft_iroot(unsigned radicand, unsigned index) { return iroot(radicand,
 
 int bbfft_is_prime(unsigned n) { return is_prime(n) ? 1 : 0; }
 
+int bbfft_root_of_unity(long k, long n, int precision, double *re, double *im) {
+    return guarded([&] {
+        require(re && im && n > 0 && (precision == 4 || precision == 8), "invalid argument");
+        if (precision == 4) {
+            auto w = gen::root_of_unity_f(k, n);
+            *re = w.real();
+            *im = w.imag();
+        } else {
+            auto w = gen::root_of_unity(k, n);
+            *re = w.real();
+            *im = w.imag();
+        }
+    });
+}
+
 int bbfft_num_passes(const bbfft_configuration *cfg, unsigned *num) {
     return guarded([&] {
         require(cfg && num, "null argument");
@@ -247,13 +261,28 @@ int bbfft_generate_source(const bbfft_configuration *cfg, unsigned pass, const c
         configuration pc = {};
         auto kc = kernel_config_of(cfg, pass, device_info, &pc);
         std::ostringstream os;
-        generate_fft1d(os, kc);
+        generate_fft1d(os, kc, {}, &pc.callbacks);
         auto s = os.str();
         if (needed) {
             *needed = s.size() + 1;
         }
         if (buf) {
             copy_out(s, buf, len);
+        }
+    });
+}
+
+int bbfft_compile_kernel(const bbfft_configuration *cfg, unsigned pass, const char *device_info,
+                         const char *arch, size_t *code_size) {
+    return guarded([&] {
+        require(arch != nullptr, "null arch");
+        configuration pc = {};
+        auto kc = kernel_config_of(cfg, pass, device_info, &pc);
+        std::ostringstream os;
+        generate_fft1d(os, kc, {}, &pc.callbacks);
+        auto code = hip::compile_to_native(os.str(), arch);
+        if (code_size) {
+            *code_size = code.size();
         }
     });
 }
@@ -353,69 +382,43 @@ int bbfft_device_info_query(int device, char *buf, size_t len) {
 }
 
 int bbfft_aot_generate(const char *const *descriptors, size_t num_descriptors,
-                       const char *device_info, const char *arch, const char *out_path, int jobs,
-                       size_t *num_kernels) {
+                       const char *device_info, const char *arch, const char *out_path,
+                       unsigned shard, unsigned num_shards, size_t *num_kernels) {
     return guarded([&] {
         require(descriptors && device_info && arch && out_path, "null argument");
+        require(num_shards > 0 && shard < num_shards, "invalid shard");
         auto info = parse_device_info(device_info);
-        // Unique kernels in first-seen order.
+        // Unique kernels in first-seen order; this shard takes every num_shards-th one.
         std::vector<std::pair<std::string, std::string>> kernels; // (name, source)
         std::set<std::string> seen;
+        std::size_t index = 0;
         for (size_t i = 0; i < num_descriptors; ++i) {
             auto cfg = parse_fft_descriptor(descriptors[i]);
             for (auto const &p : decompose(cfg).passes) {
                 auto kc = configure_fft1d(p, info);
                 auto id = kc.identifier();
                 if (seen.insert(id).second) {
-                    std::ostringstream os;
-                    generate_fft1d(os, kc);
-                    kernels.emplace_back(id, os.str());
-                }
-            }
-        }
-        // Several kernels per code object keeps the archive small and hiprtc calls few; chunks
-        // compile in parallel.
-        std::size_t const per_object = 8;
-        std::size_t const nobj = (kernels.size() + per_object - 1) / per_object;
-        std::vector<hip::archive_entry> entries(nobj);
-        std::atomic<std::size_t> next{0};
-        std::mutex err_mtx;
-        std::string first_error;
-        auto worker = [&] {
-            for (;;) {
-                std::size_t o = next++;
-                if (o >= nobj) {
-                    return;
-                }
-                std::string src;
-                auto &e = entries[o];
-                e.arch = arch;
-                for (std::size_t k = o * per_object;
-                     k < std::min(kernels.size(), (o + 1) * per_object); ++k) {
-                    src += kernels[k].second;
-                    e.kernel_names.push_back(kernels[k].first);
-                }
-                try {
-                    e.code = hip::compile_to_native(src, arch);
-                } catch (std::exception const &ex) {
-                    std::lock_guard<std::mutex> lock(err_mtx);
-                    if (first_error.empty()) {
-                        first_error = ex.what();
+                    if (index++ % num_shards == shard) {
+                        std::ostringstream os;
+                        generate_fft1d(os, kc);
+                        kernels.emplace_back(id, os.str());
                     }
                 }
             }
-        };
-        int nt = std::max(1, std::min<int>(jobs, static_cast<int>(nobj)));
-        std::vector<std::thread> threads;
-        for (int t = 1; t < nt; ++t) {
-            threads.emplace_back(worker);
         }
-        worker();
-        for (auto &t : threads) {
-            t.join();
-        }
-        if (!first_error.empty()) {
-            throw hip::error(first_error, 0);
+        // Several kernels per code object keeps the archive and the hiprtc call count small.
+        std::size_t const per_object = 16;
+        std::vector<hip::archive_entry> entries;
+        for (std::size_t k0 = 0; k0 < kernels.size(); k0 += per_object) {
+            hip::archive_entry e;
+            e.arch = arch;
+            std::string src;
+            for (std::size_t k = k0; k < std::min(kernels.size(), k0 + per_object); ++k) {
+                src += kernels[k].second;
+                e.kernel_names.push_back(kernels[k].first);
+            }
+            e.code = hip::compile_to_native(src, arch);
+            entries.push_back(std::move(e));
         }
         hip::write_archive(out_path, entries);
         if (num_kernels) {

@@ -174,6 +174,15 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
 
     std::size_t const kcols = c.K_columns(K);
     std::size_t const lds_limit = std::min<std::size_t>(info.local_memory_size, 64 * 1024);
+    if (!tuning.Mb) {
+        // Shrink the M tile until one K-column tile fits the LDS budget.
+        c.Kb = 1;
+        std::size_t mg = (M + c.Mb - 1) / c.Mb;
+        while (c.Mb > 1 && c.lds_bytes() > lds_limit) {
+            ++mg;
+            c.Mb = (M + mg - 1) / mg;
+        }
+    }
     if (tuning.Kb) {
         c.Kb = tuning.Kb;
     } else {

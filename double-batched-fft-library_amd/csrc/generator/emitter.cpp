@@ -688,12 +688,38 @@ std::string hip_prelude() {
     return os.str();
 }
 
-void generate_fft1d(std::ostream &os, fft1d_configuration const &cfg, std::string_view name) {
+namespace {
+std::string callback_source(user_module const &um) {
+    if (!um) {
+        return {};
+    }
+    std::string src(um.data, um.length);
+    if (um.language == kernel_language::opencl_c) {
+        return "#define global\n#define __global\n#define kernel\n#define __kernel\n"
+               "typedef unsigned int uint;\ntypedef unsigned long ulong;\n"
+               "typedef unsigned short ushort;\ntypedef unsigned char uchar;\n"
+               "#pragma clang force_cuda_host_device begin\n" +
+               src +
+               "\n#pragma clang force_cuda_host_device end\n"
+               "#undef global\n#undef __global\n#undef kernel\n#undef __kernel\n";
+    }
+    return src + "\n";
+}
+} // namespace
+
+void generate_fft1d(std::ostream &os, fft1d_configuration const &cfg, std::string_view name,
+                    user_module const *callbacks) {
     if (cfg.factorization.empty()) {
         throw bad_configuration("empty factorization");
     }
+    if ((cfg.load_function || cfg.store_function) && !(callbacks && *callbacks)) {
+        throw bad_configuration("callback names given without callback source");
+    }
     std::string n = name.empty() ? cfg.identifier() : std::string(name);
     os << hip_prelude();
+    if (callbacks) {
+        os << callback_source(*callbacks);
+    }
     kernel_gen g(cfg, n);
     os << g.generate();
 }

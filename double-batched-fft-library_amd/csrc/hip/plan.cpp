@@ -27,26 +27,6 @@ namespace {
 
 void release_module(module_handle_t m) { (void)hipModuleUnload(reinterpret_cast<hipModule_t>(m)); }
 
-/// User callback source as it is placed in front of the generated kernel. OpenCL-C callbacks
-/// (the reference's only language) get address-space qualifiers mapped away and are compiled as
-/// host+device functions.
-std::string callback_source(user_module const &um) {
-    if (!um) {
-        return {};
-    }
-    std::string src(um.data, um.length);
-    if (um.language == kernel_language::opencl_c) {
-        return "#define global\n#define __global\n#define kernel\n#define __kernel\n"
-               "typedef unsigned int uint;\ntypedef unsigned long ulong;\n"
-               "typedef unsigned short ushort;\ntypedef unsigned char uchar;\n"
-               "#pragma clang force_cuda_host_device begin\n" +
-               src +
-               "\n#pragma clang force_cuda_host_device end\n"
-               "#undef global\n#undef __global\n#undef kernel\n#undef __kernel\n";
-    }
-    return src + "\n";
-}
-
 } // namespace
 
 namespace hip_detail {
@@ -63,8 +43,7 @@ fft1d_plan::fft1d_plan(configuration const &cfg, hipStream_t stream, int device,
     }
     if (!mod_) {
         std::ostringstream src;
-        src << callback_source(cfg.callbacks);
-        generate_fft1d(src, kc_);
+        generate_fft1d(src, kc_, {}, &cfg.callbacks);
         hipModule_t m = hip::build_kernel_bundle(src.str(), device);
         mod_ = shared_handle<module_handle_t>(reinterpret_cast<module_handle_t>(m),
                                               &release_module);

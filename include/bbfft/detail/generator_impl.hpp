@@ -71,9 +71,16 @@ struct BBFFT_EXPORT fft1d_tuning {
 BBFFT_EXPORT fft1d_configuration configure_fft1d(configuration const &cfg,
                                                  device_info const &info,
                                                  fft1d_tuning const &tuning = {});
-/// Emit HIP source (no includes; hiprtc provides the HIP device builtins) for one kernel.
+/**
+ * @brief Emit HIP source (no includes; hiprtc provides the HIP device builtins) for one kernel.
+ *
+ * When `callbacks` is given its source is placed in front of the kernel; OpenCL-C callbacks
+ * (the reference's language) get the address-space qualifiers mapped away and are compiled as
+ * host+device functions.
+ */
 BBFFT_EXPORT void generate_fft1d(std::ostream &os, fft1d_configuration const &cfg,
-                                 std::string_view name = {});
+                                 std::string_view name = {},
+                                 user_module const *callbacks = nullptr);
 
 } // namespace bbfft
 

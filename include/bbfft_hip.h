@@ -90,6 +90,11 @@ BBFFT_EXPORT_C int bbfft_factor(unsigned n, unsigned index, unsigned *factors);
 BBFFT_EXPORT_C unsigned bbfft_iroot(unsigned radicand, unsigned index);
 BBFFT_EXPORT_C int bbfft_is_prime(unsigned n);
 
+/* exp(2 pi i k / n) as the generator emits it: correctly rounded to double (precision 8) or
+ * float (precision 4) (plays the role of power_of_w / w_lut_N, src/base/root_of_unity.cpp:144-189
+ * and src/base/root_of_unity.hpp:16-1162). */
+BBFFT_EXPORT_C int bbfft_root_of_unity(long k, long n, int precision, double *re, double *im);
+
 /* Kernel configuration and HIP source of the 1-D pass `pass` of cfg for a device-info string
  * (configure_* + generate_*, include/bbfft/detail/generator_impl.hpp:51-107 of the reference).
  * No GPU is needed. bbfft_generate_source writes at most len bytes and sets *needed. */
@@ -101,6 +106,12 @@ BBFFT_EXPORT_C int bbfft_kernel_configuration(const bbfft_configuration *cfg, un
 BBFFT_EXPORT_C int bbfft_generate_source(const bbfft_configuration *cfg, unsigned pass,
                                          const char *device_info, char *buf, size_t len,
                                          size_t *needed);
+
+/* Compile the kernel of pass `pass` (callbacks included) with hiprtc for `arch`; no GPU needed.
+ * The compiler log of a failure is returned by bbfft_last_error(). */
+BBFFT_EXPORT_C int bbfft_compile_kernel(const bbfft_configuration *cfg, unsigned pass,
+                                        const char *device_info, const char *arch,
+                                        size_t *code_size);
 
 /* Module caches: jit_cache_all (include/bbfft/jit_cache_all.hpp:20-37) layered over an
  * aot_cache (include/bbfft/aot_cache.hpp:21-48). bbfft_cache_load_aot registers an archive
@@ -133,11 +144,14 @@ BBFFT_EXPORT_C void bbfft_plan_destroy(bbfft_plan_t plan);
 BBFFT_EXPORT_C int bbfft_device_info_query(int device, char *buf, size_t len);
 
 /* bbfft-aot-generate (tools/aot/main.cpp:22-60): generate every kernel the descriptors need for
- * device_info, compile them with hiprtc for `arch` (e.g. "gfx950") using `jobs` threads and
- * write an archive to out_path. No GPU is needed. */
+ * device_info, compile them with hiprtc for `arch` (e.g. "gfx950") and write an archive to
+ * out_path. Kernel i of the de-duplicated list is compiled only when i % num_shards == shard, so
+ * independent processes can build disjoint archives in parallel (hiprtc compiles one program
+ * at a time per process). No GPU is needed. */
 BBFFT_EXPORT_C int bbfft_aot_generate(const char *const *descriptors, size_t num_descriptors,
                                       const char *device_info, const char *arch,
-                                      const char *out_path, int jobs, size_t *num_kernels);
+                                      const char *out_path, unsigned shard, unsigned num_shards,
+                                      size_t *num_kernels);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,70 @@
+"""Load/store callbacks on the GPU (reference: test/callback.cpp), compared bitwise."""
+import numpy as np
+import pytest
+
+import bbfft_amd as bb
+import callback_sources as cbs
+import oracle
+from gpu_util import execute, have_gpu
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not have_gpu(), reason="needs a HIP GPU")]
+
+CASES = [(fp, M, N) for fp in (4, 8) for M in (1, 32) for N in (8, 64, 212)]
+
+
+def _real_t(fp):
+    return "float" if fp == 4 else "double"
+
+
+@pytest.mark.parametrize("fp,M,N", CASES)
+def test_load_callback_zero_padding(fp, M, N):
+    """c2r of 2N with a zero-padding load callback == c2r of explicitly zero-padded data."""
+    K = 16
+    N_ext = 2 * N
+    n_ref, n_have = N_ext // 2 + 1, N // 2 + 1
+    ref_istride = [1, M, M * n_ref]
+    ostride = [1, M, M * N_ext]
+    cfg_ref = bb.configuration(1, [M, N_ext, K], bb.precision(fp), bb.direction.backward,
+                               bb.transform_type.c2r, ref_istride, ostride)
+    cfg = bb.configuration(1, [M, N_ext, K], bb.precision(fp), bb.direction.backward,
+                           bb.transform_type.c2r, ref_istride, ostride,
+                           bb.user_module(cbs.load_hip(_real_t(fp), M, n_ref, n_have), "load"))
+    X = oracle.uniform_complex(21, M * n_have * K).reshape(K, n_have, M)
+    X_ref = np.zeros((K, n_ref, M), dtype=np.complex128)
+    X_ref[:, :n_have, :] = X
+    x_ref = execute(cfg_ref, X_ref.ravel())
+    # the input buffer only holds the compact N/2+1 bins; pad it to the plan's extent
+    compact = np.zeros(X_ref.size, dtype=np.complex128)
+    compact[:X.size] = X.ravel()
+    x = execute(cfg, compact)
+    assert np.array_equal(x, x_ref)
+
+
+@pytest.mark.parametrize("lang", ["hip", "opencl_c"])
+@pytest.mark.parametrize("fp,M,N", CASES)
+def test_store_callback_truncate_scale(fp, M, N, lang):
+    """r2c with a store callback that keeps N/4 bins scaled by 1/N == r2c output * T(1/N)."""
+    K = 16
+    n_ref, n_cut = N // 2 + 1, N // 4
+    istride = [1, M, M * N]
+    ref_ostride = [1, M, M * n_ref]
+    real_t = _real_t(fp)
+    scale = float(np.float32(1.0 / N)) if fp == 4 else 1.0 / N
+    src = (cbs.store_hip if lang == "hip" else cbs.store_opencl)(real_t, M, n_ref, n_cut,
+                                                                 scale.hex())
+    language = bb.kernel_language.hip if lang == "hip" else bb.kernel_language.opencl_c
+    cfg_ref = bb.configuration(1, [M, N, K], bb.precision(fp), bb.direction.forward,
+                               bb.transform_type.r2c, istride, ref_ostride)
+    cfg = bb.configuration(1, [M, N, K], bb.precision(fp), bb.direction.forward,
+                           bb.transform_type.r2c, istride, ref_ostride,
+                           bb.user_module(src, None, "store", language))
+    x = oracle.uniform_real(22, M * N * K)
+    X_ref = execute(cfg_ref, x).reshape(K, n_ref, M)
+    # the store callback writes a (M, n_cut, K) tensor into the output buffer
+    real_np = np.float32 if fp == 4 else np.float64
+    X = execute(cfg, x)[:M * n_cut * K].reshape(K, n_cut, M)
+    expect_re = (X_ref.real[:, :n_cut, :].astype(real_np) * real_np(scale))
+    expect_im = (X_ref.imag[:, :n_cut, :].astype(real_np) * real_np(scale))
+    assert np.array_equal(X.real, expect_re)
+    assert np.array_equal(X.imag, expect_im)
