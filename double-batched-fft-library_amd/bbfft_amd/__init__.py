@@ -21,7 +21,9 @@ __all__ = [
     "precision", "direction", "transform_type", "kernel_language", "configuration",
     "user_module", "parse_fft_descriptor", "default_istride", "default_ostride", "make_plan",
     "Plan", "Cache", "BadConfiguration", "BbfftError", "DescriptorError", "kernel_configuration",
-    "generate_source", "aot_generate", "device_info_query", "MAX_FFT_DIM", "MAX_TENSOR_DIM",
+    "generate_source", "aot_generate", "aot_generate_shard", "device_info_query", "MAX_FFT_DIM",
+    "MAX_TENSOR_DIM", "default_cache", "root_of_unity", "factor", "num_passes", "compile_kernel",
+    "normalize_device_info", "GFX950_INFO",
 ]
 
 MAX_FFT_DIM = 3

@@ -54,6 +54,7 @@ class CConfiguration(ctypes.Structure):
 class CKernelConfig(ctypes.Structure):
     _fields_ = [
         ("M", ctypes.c_size_t), ("Mb", ctypes.c_size_t), ("N", ctypes.c_size_t),
+        ("K", ctypes.c_size_t),
         ("Nt", ctypes.c_size_t), ("Kb", ctypes.c_size_t), ("lds_pad", ctypes.c_size_t),
         ("lds_bytes", ctypes.c_size_t), ("work_group_size", ctypes.c_size_t),
         ("factorization", ctypes.c_int * 8), ("num_factors", ctypes.c_int),

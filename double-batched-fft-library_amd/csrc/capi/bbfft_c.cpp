@@ -144,6 +144,7 @@ fft1d_configuration kernel_config_of(bbfft_configuration const *cfg, unsigned pa
     require(pass < layout.passes.size(), "pass index out of range");
     if (pass_cfg) {
         *pass_cfg = layout.passes[pass];
+        pass_cfg->callbacks = c.callbacks;
     }
     return configure_fft1d(layout.passes[pass], parse_device_info(device_info));
 }
@@ -234,9 +235,11 @@ int bbfft_kernel_configuration(const bbfft_configuration *cfg, unsigned pass,
                                size_t name_len) {
     return guarded([&] {
         require(out != nullptr, "null output");
-        auto kc = kernel_config_of(cfg, pass, device_info, nullptr);
+        configuration pc = {};
+        auto kc = kernel_config_of(cfg, pass, device_info, &pc);
         std::memset(out, 0, sizeof(*out));
         out->M = kc.M;
+        out->K = pc.shape[2];
         out->Mb = kc.Mb;
         out->N = kc.N;
         out->Nt = kc.Nt;

@@ -151,9 +151,10 @@ std::shared_ptr<plan_base> create(configuration const &cfg, hipStream_t stream,
     if (cfg.dim < 1 || cfg.dim > max_fft_dim) {
         throw bad_configuration("Unsupported FFT dimension: " + std::to_string(cfg.dim));
     }
+    // Host-side validation first: configuration errors never depend on the device.
+    auto layout = decompose(cfg);
     int device = 0;
     BBFFT_HIP_CHECK(hipGetDevice(&device));
-    auto layout = decompose(cfg);
     if (layout.passes.size() == 1) {
         return std::make_shared<fft1d_plan>(layout.passes[0], stream, device, cache);
     }

@@ -22,6 +22,18 @@ pass_layout decompose(configuration const &cfg) {
     if (cfg.dim < 1 || cfg.dim > max_fft_dim) {
         throw bad_configuration("Unsupported FFT dimension: " + std::to_string(cfg.dim));
     }
+    if (cfg.istride[0] != 1 || cfg.ostride[0] != 1) {
+        throw bad_configuration("Stride s_0 != 1 is not supported");
+    }
+    if ((cfg.type == transform_type::r2c && cfg.dir != direction::forward) ||
+        (cfg.type == transform_type::c2r && cfg.dir != direction::backward)) {
+        throw bad_configuration("r2c must be forward and c2r must be backward");
+    }
+    for (unsigned d = 0; d < cfg.dim + 2; ++d) {
+        if (cfg.shape[d] == 0) {
+            throw bad_configuration("Tensor shape must not contain zeros");
+        }
+    }
     pass_layout out;
     if (cfg.dim == 1) {
         configuration c = cfg;

@@ -60,7 +60,7 @@ typedef struct bbfft_configuration {
  * replaces the reference's small_batch_configuration / factor2_slm_configuration,
  * include/bbfft/detail/generator_impl.hpp:26-107 of the reference). */
 typedef struct bbfft_kernel_config {
-    size_t M, Mb, N, Nt, Kb, lds_pad, lds_bytes, work_group_size;
+    size_t M, Mb, N, K, Nt, Kb, lds_pad, lds_bytes, work_group_size;
     int factorization[8];
     int num_factors;
     int inplace_unsupported;

@@ -193,11 +193,20 @@ void cpu_power_of_w(long k, long n, double *re, double *im) {
         k += n;
     }
     if (n <= 64) {
+        /* correctly rounded: exact on the axes, long double otherwise (|k| <= n/2 here) */
         long kp = k < 0 ? -k : k;
-        const long double tau = 6.283185307179586476925286766559005768L;
-        long double a = tau * (long double)kp / (long double)n;
-        *re = (double)cosl(a);
-        *im = (double)sinl(a);
+        if (4 * kp == n) {
+            *re = 0.0;
+            *im = 1.0;
+        } else if (2 * kp == n) {
+            *re = -1.0;
+            *im = 0.0;
+        } else {
+            const long double tau = 6.283185307179586476925286766559005768L;
+            long double a = tau * (long double)kp / (long double)n;
+            *re = (double)cosl(a);
+            *im = (double)sinl(a);
+        }
         if (k < 0) {
             *im = -*im;
         }

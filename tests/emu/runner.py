@@ -14,10 +14,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _BUILD = os.path.join(_HERE, "_build")
 
 
-def compile_kernel(source, kernel_name, in_type, out_type, callback_source=""):
+def compile_kernel(source, kernel_name, in_type, out_type):
     """Compile `source` with the emulation shim; returns a ctypes launcher."""
     os.makedirs(_BUILD, exist_ok=True)
-    tu = ('#include "hip_emu.hpp"\n' + callback_source + "\n" + source + "\n"
+    tu = ('#include "hip_emu.hpp"\n' + source + "\n"
           'extern "C" int emu_launch(const void* in, void* out, unsigned long K, unsigned grid, '
           "unsigned block) {\n"
           f"  return emu::launch(grid, block, [&] {{ {kernel_name}(({in_type}*)in, "
@@ -39,27 +39,33 @@ def compile_kernel(source, kernel_name, in_type, out_type, callback_source=""):
     return fn
 
 
-def run(bbfft, cfg, x, out_size, grid_cap=None, device_info=None, callback_source=""):
-    """Execute the 1-D configuration `cfg` (bbfft_amd.configuration) on host arrays.
+def run(bbfft, cfg, x, out_size, grid_cap=None, device_info=None, pass_index=0,
+        in_place_buffer=None):
+    """Execute 1-D pass `pass_index` of cfg on host arrays through the emulator.
 
-    x: numpy array (complex64/128 or float32/64) holding the input tensor;
-    returns a numpy array of out_size elements (complex or real per the transform type)."""
+    x: input array (complex or real per the pass); returns out_size output elements. With
+    in_place_buffer the kernel reads and writes that buffer (in == out)."""
     info = device_info or bbfft.GFX950_INFO
-    kc = bbfft.kernel_configuration(cfg, 0, info)
-    src = bbfft.generate_source(cfg, 0, info)
+    kc = bbfft.kernel_configuration(cfg, pass_index, info)
+    src = bbfft.generate_source(cfg, pass_index, info)
     f32 = int(cfg.fp) == 4
     real_t, cplx_t = ("float", "float2") if f32 else ("double", "double2")
     npc, npr = (np.complex64, np.float32) if f32 else (np.complex128, np.float64)
-    t = int(cfg.type)
+    t = pass_type(cfg, pass_index)
     in_t = real_t if t == 1 else cplx_t
     out_t = real_t if t == 2 else cplx_t
-    fn = compile_kernel(src, kc["name"], in_t, out_t, callback_source)
-    M, N, K = cfg.shape[0], cfg.shape[1], cfg.shape[2]
+    fn = compile_kernel(src, kc["name"], in_t, out_t)
+    M, N, K = kc["M"], kc["N"], kc["K"]
     pairs = t != 0 and N % 2 == 1
     kcols = (K + 1) // 2 if pairs else K
-    mg = -(-M // kc["Mb"])
-    ntiles = mg * (-(-kcols // kc["Kb"]))
+    ntiles = -(-M // kc["Mb"]) * (-(-kcols // kc["Kb"]))
     grid = ntiles if grid_cap is None else min(ntiles, grid_cap)
+    if in_place_buffer is not None:
+        rc = fn(in_place_buffer.ctypes.data, in_place_buffer.ctypes.data, K, grid,
+                kc["work_group_size"])
+        if rc != 0:
+            raise RuntimeError("divergent barrier in emulated kernel")
+        return in_place_buffer
     xin = np.ascontiguousarray(x, dtype=npr if t == 1 else npc)
     out = np.zeros(out_size, dtype=npr if t == 2 else npc)
     rc = fn(xin.ctypes.data, out.ctypes.data, K, grid, kc["work_group_size"])
@@ -68,22 +74,11 @@ def run(bbfft, cfg, x, out_size, grid_cap=None, device_info=None, callback_sourc
     return out
 
 
-def run_inplace(bbfft, cfg, buf, grid_cap=None, device_info=None):
-    """In-place execution on one host buffer (numpy array of the input element type)."""
-    info = device_info or bbfft.GFX950_INFO
-    kc = bbfft.kernel_configuration(cfg, 0, info)
-    src = bbfft.generate_source(cfg, 0, info)
-    f32 = int(cfg.fp) == 4
-    real_t, cplx_t = ("float", "float2") if f32 else ("double", "double2")
+def pass_type(cfg, pass_index):
+    """Transform type of a 1-D pass: only the real pass (first for r2c, last for c2r) is real."""
     t = int(cfg.type)
-    fn = compile_kernel(src, kc["name"], real_t if t == 1 else cplx_t,
-                        real_t if t == 2 else cplx_t)
-    M, N, K = cfg.shape[0], cfg.shape[1], cfg.shape[2]
-    pairs = t != 0 and N % 2 == 1
-    kcols = (K + 1) // 2 if pairs else K
-    ntiles = -(-M // kc["Mb"]) * (-(-kcols // kc["Kb"]))
-    grid = ntiles if grid_cap is None else min(ntiles, grid_cap)
-    rc = fn(buf.ctypes.data, buf.ctypes.data, K, grid, kc["work_group_size"])
-    if rc != 0:
-        raise RuntimeError("divergent barrier in emulated kernel")
-    return buf
+    if t == 0 or cfg.dim == 1:
+        return t
+    if t == 1:
+        return 1 if pass_index == 0 else 0
+    return 2 if pass_index == cfg.dim - 1 else 0

@@ -17,7 +17,7 @@ def main(ref):
     text = open(os.path.join(ref, "src/base/root_of_unity.hpp")).read()
     out = {}
     for n in NS:
-        m = re.search(r"w_lut_%d\s*=\s*\{\{(.*?)\}\};" % n, text, re.S)
+        m = re.search(r"w_lut_%d\s*=\s*\{\s*\{(.*?)\}\};" % n, text, re.S)
         if not m:
             raise SystemExit(f"table w_lut_{n} not found")
         pairs = re.findall(r"\{\s*([-+0-9a-fx.p]+)\s*,\s*([-+0-9a-fx.p]+)\s*\}", m.group(1))

@@ -1,0 +1,135 @@
+"""The generated HIP kernels, compiled for the host and run by the fiber emulator
+(tests/emu/hip_emu.hpp), against the FP64 oracle. CPU only: covers the generator's indexing,
+LDS exchange, barriers, tile loop, real pre/post passes, callbacks and nd pass chaining."""
+import numpy as np
+import pytest
+
+import bbfft_amd as bb
+import callback_sources as cbs
+import oracle
+from emu import runner
+
+
+def _cfg(M, N, K, fp=4, d=-1, t=0, inplace=False, dims=None):
+    dims = dims or [N]
+    cfg = bb.configuration(len(dims), [M] + dims + [K], bb.precision(fp), bb.direction(d),
+                           bb.transform_type(t))
+    cfg.set_strides_default(inplace)
+    return cfg
+
+
+@pytest.mark.parametrize("fp", [4, 8])
+@pytest.mark.parametrize("N", [2, 3, 7, 8, 12, 16, 17, 32, 60, 64, 105, 128, 243, 256, 384, 512])
+def test_c2c(N, fp):
+    for M, K, d, cap in [(3, 5, -1, 2), (8, 3, 1, None)]:
+        cfg = _cfg(M, N, K, fp, d)
+        x = oracle.uniform_complex(N, M * N * K)
+        y = runner.run(bb, cfg, x, M * N * K, grid_cap=cap)
+        ref = oracle.fft(1, cfg.shape, cfg.istride, cfg.ostride, 0, d, x)
+        assert oracle.rel_l2(y, ref) <= oracle.tolerance(fp, N)
+
+
+@pytest.mark.parametrize("N", [2, 5, 8, 10, 26, 27, 64, 102, 105, 220, 256])
+@pytest.mark.parametrize("inplace", [False, True])
+def test_r2c(N, inplace):
+    for M, K, fp in [(1, 5, 4), (3, 4, 8)]:
+        cfg = _cfg(M, N, K, fp, -1, 1, inplace)
+        n_in = oracle.extent(1, cfg.shape, cfg.istride)
+        n_out = oracle.extent(1, [M, N // 2 + 1, K], cfg.ostride)
+        x = oracle.uniform_real(N, n_in)
+        ref = oracle.fft(1, cfg.shape, cfg.istride, cfg.ostride, 1, -1, x, out_size=n_out)
+        if inplace:
+            rt = np.float32 if fp == 4 else np.float64
+            buf = np.zeros(max(n_in, 2 * n_out), rt)
+            buf[:n_in] = x
+            y = runner.run(bb, cfg, None, 0, in_place_buffer=buf).view(
+                np.complex64 if fp == 4 else np.complex128)[:n_out]
+        else:
+            y = runner.run(bb, cfg, x, n_out, grid_cap=2)
+        assert oracle.rel_l2(y, ref) <= oracle.tolerance(fp, N)
+
+
+@pytest.mark.parametrize("N", [2, 3, 11, 16, 21, 25, 48, 96, 300, 315, 512])
+@pytest.mark.parametrize("inplace", [False, True])
+def test_c2r(N, inplace):
+    """DC and Nyquist imaginary parts are non-zero in the inputs (reference rules)."""
+    for M, K, fp in [(1, 5, 4), (5, 3, 8)]:
+        cfg = _cfg(M, N, K, fp, 1, 2, inplace)
+        n_in = oracle.extent(1, [M, N // 2 + 1, K], cfg.istride)
+        n_out = oracle.extent(1, cfg.shape, cfg.ostride)
+        X = oracle.uniform_complex(N, n_in)
+        ref = oracle.fft(1, cfg.shape, cfg.istride, cfg.ostride, 2, 1, X, out_size=n_out)
+        if inplace:
+            ct = np.complex64 if fp == 4 else np.complex128
+            buf = np.zeros(max(n_in, (n_out + 1) // 2), ct)
+            buf[:n_in] = X
+            y = runner.run(bb, cfg, None, 0, in_place_buffer=buf).view(
+                np.float32 if fp == 4 else np.float64)[:n_out]
+        else:
+            y = runner.run(bb, cfg, X, n_out, grid_cap=2)
+        m, n, k = np.meshgrid(range(M), range(N), range(K), indexing="ij")
+        sel = (m + n * cfg.ostride[1] + k * cfg.ostride[2]).ravel()
+        assert oracle.rel_l2(y[sel], ref[sel]) <= oracle.tolerance(fp, N)
+
+
+@pytest.mark.parametrize("dims,t", [([8, 5], 0), ([5, 6, 4], 0), ([8, 6], 1), ([5, 4, 6], 1),
+                                    ([8, 6], 2), ([5, 4, 6], 2)])
+def test_nd_pass_chain(dims, t):
+    """2-D/3-D transforms as the plan chains them: pass 0 in->out, later passes out->out."""
+    M, K = 2, 3
+    cfg = _cfg(M, 0, K, 4, -1 if t != 2 else 1, t, False, dims)
+    shape = cfg.shape
+    half = list(shape)
+    half[1] = dims[0] // 2 + 1
+    D = len(dims)
+    ishape, oshape = (shape, half) if t == 1 else (half, shape) if t == 2 else (shape, shape)
+    n_in = oracle.extent(D, ishape, cfg.istride)
+    n_out = oracle.extent(D, oshape, cfg.ostride)
+    x = oracle.uniform_real(3, n_in) if t == 1 else oracle.uniform_complex(3, n_in)
+    ref = oracle.fft(D, shape, cfg.istride, cfg.ostride, t, int(cfg.dir), x, out_size=n_out)
+    assert bb.num_passes(cfg) == D
+    cur = runner.run(bb, cfg, x, n_out, pass_index=0)
+    for p in range(1, D):
+        cur = runner.run(bb, cfg, None, 0, pass_index=p, in_place_buffer=cur) if p < D - 1 \
+            else runner.run(bb, cfg, cur, n_out, pass_index=p)
+    assert oracle.rel_l2(cur, ref) <= oracle.tolerance(4, int(np.prod(dims)))
+
+
+@pytest.mark.parametrize("fp,M,N", [(4, 1, 8), (4, 4, 64), (8, 3, 212)])
+def test_callbacks_bitwise(fp, M, N):
+    """Zero-padding load and truncating/scaling store callbacks equal the plain plans bitwise
+    (reference test/callback.cpp)."""
+    K = 4
+    real_t = "float" if fp == 4 else "double"
+    N_ext = 2 * N
+    n_ref, n_have = N_ext // 2 + 1, N // 2 + 1
+    plain = bb.configuration(1, [M, N_ext, K], bb.precision(fp), bb.direction.backward,
+                             bb.transform_type.c2r, [1, M, M * n_ref], [1, M, M * N_ext])
+    cb = bb.configuration(1, [M, N_ext, K], bb.precision(fp), bb.direction.backward,
+                          bb.transform_type.c2r, [1, M, M * n_ref], [1, M, M * N_ext],
+                          bb.user_module(cbs.load_hip(real_t, M, n_ref, n_have), "load"))
+    X = oracle.uniform_complex(5, M * n_have * K).reshape(K, n_have, M)
+    Xp = np.zeros((K, n_ref, M), complex)
+    Xp[:, :n_have] = X
+    compact = np.zeros(Xp.size, complex)
+    compact[:X.size] = X.ravel()
+    a = runner.run(bb, plain, Xp.ravel(), M * N_ext * K)
+    b = runner.run(bb, cb, compact, M * N_ext * K)
+    assert np.array_equal(a, b)
+    # store: keep N/4 bins scaled by 1/N, HIP and OpenCL-C dialects
+    n_h, n_cut = N // 2 + 1, N // 4
+    scale = float(np.float32(1.0 / N)) if fp == 4 else 1.0 / N
+    plain = bb.configuration(1, [M, N, K], bb.precision(fp), bb.direction.forward,
+                             bb.transform_type.r2c, [1, M, M * N], [1, M, M * n_h])
+    x = oracle.uniform_real(6, M * N * K)
+    ref = runner.run(bb, plain, x, M * n_h * K).reshape(K, n_h, M)
+    rt = np.float32 if fp == 4 else np.float64
+    for src, lang in [(cbs.store_hip, bb.kernel_language.hip),
+                      (cbs.store_opencl, bb.kernel_language.opencl_c)]:
+        cfg = bb.configuration(1, [M, N, K], bb.precision(fp), bb.direction.forward,
+                               bb.transform_type.r2c, [1, M, M * N], [1, M, M * n_h],
+                               bb.user_module(src(real_t, M, n_h, n_cut, scale.hex()), None,
+                                              "store", lang))
+        y = runner.run(bb, cfg, x, M * n_h * K)[:M * n_cut * K].reshape(K, n_cut, M)
+        assert np.array_equal(y.real, ref.real[:, :n_cut] * rt(scale))
+        assert np.array_equal(y.imag, ref.imag[:, :n_cut] * rt(scale))
