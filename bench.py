@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--desc", default="scfo8.256*32768")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device-to-device copy 5 times (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the double-batched FFT "
@@ -91,6 +93,11 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if a.copy_probe:
+        gbps = copy_bandwidth(torch, nbytes=1 << 30, reps=5)
+        print(json.dumps({"copy_probe_bytes": 1 << 30, "GBps": round(gbps, 1)}), flush=True)
+        return
 
     import bbfft_amd as bb
     cache = bb.default_cache()

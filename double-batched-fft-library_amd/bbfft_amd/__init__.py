@@ -203,23 +203,54 @@ def num_passes(cfg: configuration) -> int:
     return n.value
 
 
-def kernel_configuration(cfg: configuration, pass_index=0, device_info=GFX950_INFO):
+def tuning_struct(t):
+    """dict(factorization=[...], Nt=, Mb=, Kb=, lds_pad=, no_pad=, flags=) -> CTuning."""
+    c = _capi.CTuning()
+    fac = list(t.get("factorization", []))
+    c.num_factors = len(fac)
+    for i, f in enumerate(fac):
+        c.factorization[i] = f
+    for k in ("Nt", "Mb", "Kb", "lds_pad"):
+        setattr(c, k, int(t.get(k, 0)))
+    c.no_pad = int(bool(t.get("no_pad", False)))
+    if "flags" in t:
+        c.flags = int(t["flags"])
+        c.has_flags = 1
+    return c
+
+
+def kernel_configuration(cfg: configuration, pass_index=0, device_info=GFX950_INFO,
+                         tuning=None):
     """Tiling of the kernel of 1-D pass `pass_index` (dict) plus its identifier."""
     kc = _capi.CKernelConfig()
     name = ctypes.create_string_buffer(512)
-    _check(_capi.load().bbfft_kernel_configuration(ctypes.byref(cfg._to_c()), pass_index,
-                                                   device_info.encode(), ctypes.byref(kc), name,
-                                                   512))
+    if tuning is not None:
+        _check(_capi.load().bbfft_kernel_configuration_tuned(
+            ctypes.byref(cfg._to_c()), ctypes.byref(tuning_struct(tuning)),
+            device_info.encode(), ctypes.byref(kc), name, 512))
+    else:
+        _check(_capi.load().bbfft_kernel_configuration(ctypes.byref(cfg._to_c()), pass_index,
+                                                       device_info.encode(), ctypes.byref(kc),
+                                                       name, 512))
     d = {f: getattr(kc, f) for f, _ in _capi.CKernelConfig._fields_ if f != "factorization"}
     d["factorization"] = list(kc.factorization)[:kc.num_factors]
     d["name"] = name.value.decode()
     return d
 
 
-def generate_source(cfg: configuration, pass_index=0, device_info=GFX950_INFO) -> str:
+def generate_source(cfg: configuration, pass_index=0, device_info=GFX950_INFO,
+                    tuning=None) -> str:
     L = _capi.load()
     c = cfg._to_c()
     n = ctypes.c_size_t()
+    if tuning is not None:
+        t = tuning_struct(tuning)
+        _check(L.bbfft_generate_source_tuned(ctypes.byref(c), ctypes.byref(t),
+                                             device_info.encode(), None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(L.bbfft_generate_source_tuned(ctypes.byref(c), ctypes.byref(t),
+                                             device_info.encode(), buf, n.value, None))
+        return buf.value.decode()
     _check(L.bbfft_generate_source(ctypes.byref(c), pass_index, device_info.encode(), None, 0,
                                    ctypes.byref(n)))
     buf = ctypes.create_string_buffer(n.value)
@@ -371,14 +402,20 @@ def default_cache(device=0) -> "Cache":
     return _default_cache
 
 
-def make_plan(cfg, stream=None, cache: Optional[Cache] = None) -> Plan:
-    """make_plan(cfg, queue, cache) for the current HIP device; cfg may be a descriptor."""
+def make_plan(cfg, stream=None, cache: Optional[Cache] = None, tuning=None) -> Plan:
+    """make_plan(cfg, queue, cache) for the current HIP device; cfg may be a descriptor.
+    `tuning` (dict, 1-D only) overrides the tiling heuristics."""
     if isinstance(cfg, str):
         cfg = parse_fft_descriptor(cfg)
     if stream is None:
         stream = _current_stream()
     h = ctypes.c_void_p()
-    _check(_capi.load().bbfft_plan_create(ctypes.byref(cfg._to_c()), stream,
-                                          cache.handle if cache is not None else None,
-                                          ctypes.byref(h)))
+    ch = cache.handle if cache is not None else None
+    if tuning is not None:
+        _check(_capi.load().bbfft_plan_create_tuned(ctypes.byref(cfg._to_c()),
+                                                    ctypes.byref(tuning_struct(tuning)), stream,
+                                                    ch, ctypes.byref(h)))
+    else:
+        _check(_capi.load().bbfft_plan_create(ctypes.byref(cfg._to_c()), stream, ch,
+                                              ctypes.byref(h)))
     return Plan(h, cfg, stream)

@@ -17,7 +17,8 @@ EXPORTED = [
     "bbfft_device_info_normalize", "bbfft_factor", "bbfft_iroot", "bbfft_is_prime",
     "bbfft_root_of_unity",
     "bbfft_num_passes", "bbfft_kernel_configuration", "bbfft_generate_source",
-    "bbfft_compile_kernel",
+    "bbfft_compile_kernel", "bbfft_kernel_configuration_tuned", "bbfft_generate_source_tuned",
+    "bbfft_plan_create_tuned",
     "bbfft_cache_create", "bbfft_cache_load_aot", "bbfft_cache_size", "bbfft_cache_destroy",
     "bbfft_plan_create", "bbfft_plan_create_from_descriptor", "bbfft_plan_execute",
     "bbfft_plan_num_launches", "bbfft_plan_launch_info", "bbfft_plan_destroy",
@@ -62,6 +63,15 @@ class CKernelConfig(ctypes.Structure):
     ]
 
 
+class CTuning(ctypes.Structure):
+    _fields_ = [
+        ("factorization", ctypes.c_int * 8), ("num_factors", ctypes.c_int),
+        ("Nt", ctypes.c_size_t), ("Mb", ctypes.c_size_t), ("Kb", ctypes.c_size_t),
+        ("lds_pad", ctypes.c_size_t), ("no_pad", ctypes.c_int), ("flags", ctypes.c_uint),
+        ("has_flags", ctypes.c_int),
+    ]
+
+
 _lib = None
 
 
@@ -103,6 +113,12 @@ def load():
                                           P(c_size)]),
         "bbfft_compile_kernel": (c_int, [P(CConfiguration), c_uint, c_char_p, c_char_p,
                                          P(c_size)]),
+        "bbfft_kernel_configuration_tuned": (c_int, [P(CConfiguration), P(CTuning), c_char_p,
+                                                     P(CKernelConfig), c_char_p, c_size]),
+        "bbfft_generate_source_tuned": (c_int, [P(CConfiguration), P(CTuning), c_char_p,
+                                                c_char_p, c_size, P(c_size)]),
+        "bbfft_plan_create_tuned": (c_int, [P(CConfiguration), P(CTuning), c_void_p, c_void_p,
+                                            P(c_void_p)]),
         "bbfft_cache_create": (c_int, [P(c_void_p)]),
         "bbfft_cache_load_aot": (c_int, [c_void_p, c_char_p, c_int, P(c_size)]),
         "bbfft_cache_size": (c_int, [c_void_p, P(c_size)]),

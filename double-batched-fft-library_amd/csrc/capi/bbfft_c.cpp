@@ -136,8 +136,26 @@ std::array<std::size_t, max_tensor_dim> shape_of(size_t const *shape) {
     return s;
 }
 
+fft1d_tuning tuning_of(bbfft_tuning const *t) {
+    fft1d_tuning r;
+    if (!t) {
+        return r;
+    }
+    require(t->num_factors >= 0 && t->num_factors <= 8, "num_factors must be in [0, 8]");
+    r.factorization.assign(t->factorization, t->factorization + t->num_factors);
+    r.Nt = t->Nt;
+    r.Mb = t->Mb;
+    r.Kb = t->Kb;
+    r.lds_pad = t->lds_pad;
+    r.no_pad = t->no_pad != 0;
+    r.flags = t->flags;
+    r.has_flags = t->has_flags != 0;
+    return r;
+}
+
 fft1d_configuration kernel_config_of(bbfft_configuration const *cfg, unsigned pass,
-                                     char const *device_info, configuration *pass_cfg) {
+                                     char const *device_info, configuration *pass_cfg,
+                                     bbfft_tuning const *tuning = nullptr) {
     require(cfg && device_info, "null argument");
     auto c = from_c(*cfg);
     auto layout = decompose(c);
@@ -146,7 +164,9 @@ fft1d_configuration kernel_config_of(bbfft_configuration const *cfg, unsigned pa
         *pass_cfg = layout.passes[pass];
         pass_cfg->callbacks = c.callbacks;
     }
-    return configure_fft1d(layout.passes[pass], parse_device_info(device_info));
+    require(tuning == nullptr || layout.passes.size() == 1, "tuning applies to 1-D only");
+    return configure_fft1d(layout.passes[pass], parse_device_info(device_info),
+                           tuning_of(tuning));
 }
 
 } // namespace
@@ -230,6 +250,49 @@ int bbfft_num_passes(const bbfft_configuration *cfg, unsigned *num) {
     });
 }
 
+namespace {
+void fill_kernel_config(fft1d_configuration const &kc, configuration const &pc,
+                        bbfft_kernel_config *out, char *name, size_t name_len);
+}
+
+int bbfft_kernel_configuration_tuned(const bbfft_configuration *cfg, const bbfft_tuning *tuning,
+                                     const char *device_info, bbfft_kernel_config *out,
+                                     char *name, size_t name_len) {
+    return guarded([&] {
+        require(out != nullptr, "null output");
+        configuration pc = {};
+        auto kc = kernel_config_of(cfg, 0, device_info, &pc, tuning);
+        fill_kernel_config(kc, pc, out, name, name_len);
+    });
+}
+
+int bbfft_generate_source_tuned(const bbfft_configuration *cfg, const bbfft_tuning *tuning,
+                                const char *device_info, char *buf, size_t len, size_t *needed) {
+    return guarded([&] {
+        configuration pc = {};
+        auto kc = kernel_config_of(cfg, 0, device_info, &pc, tuning);
+        std::ostringstream os;
+        generate_fft1d(os, kc, {}, &pc.callbacks);
+        auto s = os.str();
+        if (needed) {
+            *needed = s.size() + 1;
+        }
+        if (buf) {
+            copy_out(s, buf, len);
+        }
+    });
+}
+
+int bbfft_plan_create_tuned(const bbfft_configuration *cfg, const bbfft_tuning *tuning,
+                            bbfft_stream_t stream, bbfft_cache_t cache, bbfft_plan_t *plan) {
+    return guarded([&] {
+        require(cfg && tuning && plan, "null argument");
+        auto impl = hip_detail::create_tuned(from_c(*cfg), tuning_of(tuning),
+                                             static_cast<hipStream_t>(stream), cache);
+        *plan = new bbfft_plan_s{std::move(impl)};
+    });
+}
+
 int bbfft_kernel_configuration(const bbfft_configuration *cfg, unsigned pass,
                                const char *device_info, bbfft_kernel_config *out, char *name,
                                size_t name_len) {
@@ -237,26 +300,33 @@ int bbfft_kernel_configuration(const bbfft_configuration *cfg, unsigned pass,
         require(out != nullptr, "null output");
         configuration pc = {};
         auto kc = kernel_config_of(cfg, pass, device_info, &pc);
-        std::memset(out, 0, sizeof(*out));
-        out->M = kc.M;
-        out->K = pc.shape[2];
-        out->Mb = kc.Mb;
-        out->N = kc.N;
-        out->Nt = kc.Nt;
-        out->Kb = kc.Kb;
-        out->lds_pad = kc.lds_pad;
-        out->lds_bytes = kc.lds_bytes();
-        out->work_group_size = kc.work_group_size();
-        out->num_factors = static_cast<int>(std::min<std::size_t>(8, kc.factorization.size()));
-        for (int i = 0; i < out->num_factors; ++i) {
-            out->factorization[i] = kc.factorization[i];
-        }
-        out->inplace_unsupported = kc.inplace_unsupported ? 1 : 0;
-        if (name) {
-            copy_out(kc.identifier(), name, name_len);
-        }
+        fill_kernel_config(kc, pc, out, name, name_len);
     });
 }
+
+namespace {
+void fill_kernel_config(fft1d_configuration const &kc, configuration const &pc,
+                        bbfft_kernel_config *out, char *name, size_t name_len) {
+    std::memset(out, 0, sizeof(*out));
+    out->M = kc.M;
+    out->K = pc.shape[2];
+    out->Mb = kc.Mb;
+    out->N = kc.N;
+    out->Nt = kc.Nt;
+    out->Kb = kc.Kb;
+    out->lds_pad = kc.lds_pad;
+    out->lds_bytes = kc.lds_bytes();
+    out->work_group_size = kc.work_group_size();
+    out->num_factors = static_cast<int>(std::min<std::size_t>(8, kc.factorization.size()));
+    for (int i = 0; i < out->num_factors; ++i) {
+        out->factorization[i] = kc.factorization[i];
+    }
+    out->inplace_unsupported = kc.inplace_unsupported ? 1 : 0;
+    if (name) {
+        copy_out(kc.identifier(), name, name_len);
+    }
+}
+} // namespace
 
 int bbfft_generate_source(const bbfft_configuration *cfg, unsigned pass, const char *device_info,
                           char *buf, size_t len, size_t *needed) {

@@ -50,6 +50,9 @@ std::string fft1d_configuration::identifier() const {
         << to_string(type) << "_is" << istride[0] << '_' << istride[1] << '_' << istride[2]
         << "_os" << ostride[0] << '_' << ostride[1] << '_' << ostride[2] << "_p" << lds_pad
         << "_in" << inplace_unsupported;
+    if (flags) {
+        oss << "_g" << flags;
+    }
     if (load_function) {
         oss << '_' << load_function;
     }
@@ -200,6 +203,7 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     if (c.lds_bytes() > info.local_memory_size) {
         throw bad_configuration("Work-group exceeds the local memory size");
     }
+    c.flags = tuning.has_flags ? tuning.flags : 0u;
     c.inplace_unsupported = cfg.type != transform_type::c2c && c.Mb < M;
     return c;
 }

@@ -301,12 +301,18 @@ class kernel_gen {
         if (c_.load_function) {
             return std::string(c_.load_function) + "((" + in_type() + "*)in, " + off + ")";
         }
+        if (c_.flags & fft1d_nontemporal_load) {
+            return "ld_nt(in + " + off + ")";
+        }
         return "in[" + off + "]";
     }
     /// Global store of `val` at offset `off` (elements from `out`).
     std::string gstore(std::string const &off, std::string const &val) const {
         if (c_.store_function) {
             return std::string(c_.store_function) + "(out, " + off + ", " + val + ");";
+        }
+        if (c_.flags & fft1d_nontemporal_store) {
+            return "st_nt(out + " + off + ", " + val + ");";
         }
         return "out[" + off + "] = " + val + ";";
     }
@@ -451,7 +457,10 @@ class kernel_gen {
             w_.line(v + " = " + gload(ioff(n)) + ";");
         } else if (half_) {
             if (vector_real_in()) {
-                w_.line(v + " = ((const " + T2() + "*)(in + ib))[" + n + "];");
+                std::string p = "((const " + T2() + "*)(in + ib)) + (" + n + ")";
+                w_.line(v + " = " + ((c_.flags & fft1d_nontemporal_load) ? "ld_nt(" + p + ")"
+                                                                          : "*(" + p + ")") +
+                        ";");
             } else {
                 w_.line(v + " = mk(" + gload(ioff("2u * (" + n + ")")) + ", " +
                         gload(ioff("2u * (" + n + ") + 1u")) + ");");
@@ -468,7 +477,9 @@ class kernel_gen {
             w_.line(gstore(ooff(n), y));
         } else if (half_) {
             if (vector_real_out()) {
-                w_.line("((" + T2() + "*)(out + ob))[" + n + "] = " + y + ";");
+                std::string p = "((" + T2() + "*)(out + ob)) + (" + n + ")";
+                w_.line((c_.flags & fft1d_nontemporal_store) ? "st_nt(" + p + ", " + y + ");"
+                                                             : "*(" + p + ") = " + y + ";");
             } else {
                 w_.line(gstore(ooff("2u * (" + n + ")"), y + ".x"));
                 w_.line(gstore(ooff("2u * (" + n + ") + 1u"), y + ".y"));
@@ -683,6 +694,17 @@ std::string hip_prelude() {
            << " a) { return mk(a.y, -a.x); }\n";
         os << "static __device__ __forceinline__ " << t2 << " c_neg(" << t2
            << " a) { return mk(-a.x, -a.y); }\n";
+        // streaming (non-temporal) global access, component-wise
+        os << "static __device__ __forceinline__ " << t2 << " ld_nt(const " << t2
+           << "* p) { return mk(__builtin_nontemporal_load(&p->x), "
+              "__builtin_nontemporal_load(&p->y)); }\n";
+        os << "static __device__ __forceinline__ void st_nt(" << t2 << "* p, " << t2
+           << " v) { __builtin_nontemporal_store(v.x, &p->x); "
+              "__builtin_nontemporal_store(v.y, &p->y); }\n";
+        os << "static __device__ __forceinline__ " << t << " ld_nt(const " << t
+           << "* p) { return __builtin_nontemporal_load(p); }\n";
+        os << "static __device__ __forceinline__ void st_nt(" << t << "* p, " << t
+           << " v) { __builtin_nontemporal_store(v, p); }\n";
     }
     os << "#endif\n";
     return os.str();

@@ -33,7 +33,8 @@ class plan_base : public detail::plan_unmanaged_event_impl<hipEvent_t> {
 /// A single 1-D double-batched kernel.
 class fft1d_plan final : public plan_base {
   public:
-    fft1d_plan(configuration const &cfg, hipStream_t stream, int device, jit_cache *cache);
+    fft1d_plan(configuration const &cfg, hipStream_t stream, int device, jit_cache *cache,
+               fft1d_tuning const &tuning = {});
     void execute(void const *in, void *out, hipEvent_t signal, std::uint32_t nwait,
                  hipEvent_t *wait) override;
     /// Launch on the plan's stream without event handling.
@@ -70,6 +71,9 @@ class nd_plan final : public plan_base {
 
 std::shared_ptr<plan_base> create(configuration const &cfg, hipStream_t stream,
                                   jit_cache *cache);
+/// 1-D plan with explicit tuning.
+std::shared_ptr<plan_base> create_tuned(configuration const &cfg, fft1d_tuning const &tuning,
+                                        hipStream_t stream, jit_cache *cache);
 
 } // namespace bbfft::hip_detail
 

@@ -32,9 +32,9 @@ void release_module(module_handle_t m) { (void)hipModuleUnload(reinterpret_cast<
 namespace hip_detail {
 
 fft1d_plan::fft1d_plan(configuration const &cfg, hipStream_t stream, int device,
-                       jit_cache *cache)
+                       jit_cache *cache, fft1d_tuning const &tuning)
     : stream_(stream), device_(device) {
-    kc_ = configure_fft1d(cfg, hip::get_device_info(device));
+    kc_ = configure_fft1d(cfg, hip::get_device_info(device), tuning);
     name_ = kc_.identifier();
     K_ = cfg.shape[2];
     jit_cache_key key{name_, hip::get_device_id(device)};
@@ -159,6 +159,17 @@ std::shared_ptr<plan_base> create(configuration const &cfg, hipStream_t stream,
         return std::make_shared<fft1d_plan>(layout.passes[0], stream, device, cache);
     }
     return std::make_shared<nd_plan>(layout, stream, device, cache);
+}
+
+std::shared_ptr<plan_base> create_tuned(configuration const &cfg, fft1d_tuning const &tuning,
+                                        hipStream_t stream, jit_cache *cache) {
+    auto layout = decompose(cfg);
+    if (layout.passes.size() != 1) {
+        throw bad_configuration("Explicit tuning applies to 1-D configurations only");
+    }
+    int device = 0;
+    BBFFT_HIP_CHECK(hipGetDevice(&device));
+    return std::make_shared<fft1d_plan>(layout.passes[0], stream, device, cache, tuning);
 }
 
 } // namespace hip_detail

@@ -40,6 +40,7 @@ struct BBFFT_EXPORT fft1d_configuration {
     std::array<std::size_t, 3u> istride; ///< {1, s_N, s_K} of the input
     std::array<std::size_t, 3u> ostride; ///< {1, s_N, s_K} of the output
     std::size_t lds_pad;                 ///< one padding slot per lds_pad slots (0 = none)
+    unsigned flags;                      ///< fft1d_flag bits (global access hints)
     bool inplace_unsupported;            ///< in == out would race across work-groups
     char const *load_function;           ///< user load callback or nullptr
     char const *store_function;          ///< user store callback or nullptr
@@ -57,6 +58,12 @@ struct BBFFT_EXPORT fft1d_configuration {
     std::size_t lds_bytes() const;
 };
 
+/// Bits of fft1d_configuration::flags.
+enum fft1d_flag : unsigned {
+    fft1d_nontemporal_load = 1u,  ///< stream the input past the caches (read once)
+    fft1d_nontemporal_store = 2u, ///< stream the output past the caches (written once)
+};
+
 /// Optional overrides of the tiling heuristics (0 / empty = choose automatically).
 struct BBFFT_EXPORT fft1d_tuning {
     std::vector<int> factorization;
@@ -65,6 +72,8 @@ struct BBFFT_EXPORT fft1d_tuning {
     std::size_t Kb = 0;
     std::size_t lds_pad = 0;
     bool no_pad = false;
+    unsigned flags = 0;
+    bool has_flags = false; ///< use `flags` instead of the default
 };
 
 /// Choose a kernel configuration for a 1-D configuration (dim == 1).

@@ -66,6 +66,18 @@ typedef struct bbfft_kernel_config {
     int inplace_unsupported;
 } bbfft_kernel_config;
 
+/* Overrides of the tiling heuristics of a 1-D plan (0 / num_factors == 0: automatic).
+ * flags: bit 0 non-temporal global loads, bit 1 non-temporal global stores; used only when
+ * has_flags != 0. (No reference counterpart: the reference's tiling is fixed by device_info.) */
+typedef struct bbfft_tuning {
+    int factorization[8];
+    int num_factors;
+    size_t Nt, Mb, Kb, lds_pad;
+    int no_pad;
+    unsigned flags;
+    int has_flags;
+} bbfft_tuning;
+
 /* Message of the last failed call on this thread ("" if none). */
 BBFFT_EXPORT_C const char *bbfft_last_error(void);
 /* Library version string. */
@@ -106,6 +118,16 @@ BBFFT_EXPORT_C int bbfft_kernel_configuration(const bbfft_configuration *cfg, un
 BBFFT_EXPORT_C int bbfft_generate_source(const bbfft_configuration *cfg, unsigned pass,
                                          const char *device_info, char *buf, size_t len,
                                          size_t *needed);
+/* Same as bbfft_kernel_configuration / bbfft_generate_source with explicit tuning (1-D). */
+BBFFT_EXPORT_C int bbfft_kernel_configuration_tuned(const bbfft_configuration *cfg,
+                                                    const bbfft_tuning *tuning,
+                                                    const char *device_info,
+                                                    bbfft_kernel_config *out, char *name,
+                                                    size_t name_len);
+BBFFT_EXPORT_C int bbfft_generate_source_tuned(const bbfft_configuration *cfg,
+                                               const bbfft_tuning *tuning,
+                                               const char *device_info, char *buf, size_t len,
+                                               size_t *needed);
 
 /* Compile the kernel of pass `pass` (callbacks included) with hiprtc for `arch`; no GPU needed.
  * The compiler log of a failure is returned by bbfft_last_error(). */
@@ -128,6 +150,10 @@ BBFFT_EXPORT_C int bbfft_plan_create(const bbfft_configuration *cfg, bbfft_strea
                                      bbfft_cache_t cache, bbfft_plan_t *plan);
 BBFFT_EXPORT_C int bbfft_plan_create_from_descriptor(const char *desc, bbfft_stream_t stream,
                                                      bbfft_cache_t cache, bbfft_plan_t *plan);
+/* make_plan of a 1-D configuration with explicit tuning (autotuner candidates). */
+BBFFT_EXPORT_C int bbfft_plan_create_tuned(const bbfft_configuration *cfg,
+                                           const bbfft_tuning *tuning, bbfft_stream_t stream,
+                                           bbfft_cache_t cache, bbfft_plan_t *plan);
 /* plan_unmanaged_event::execute (include/bbfft/plan.hpp:165-183): waits for `wait`, launches,
  * records `signal` (both optional); in == out is the in-place transform. Asynchronous. */
 BBFFT_EXPORT_C int bbfft_plan_execute(bbfft_plan_t plan, const void *in, void *out,

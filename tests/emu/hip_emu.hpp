@@ -28,6 +28,8 @@ struct emu_dim3 {
 #define __constant__
 #define __shared__ static
 #define __launch_bounds__(x)
+#define __builtin_nontemporal_load(p) (*(p))
+#define __builtin_nontemporal_store(v, p) (*(p) = (v))
 
 namespace emu {
 inline emu_dim3 threadIdx_, blockIdx_, gridDim_, blockDim_;
