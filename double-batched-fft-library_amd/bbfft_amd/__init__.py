@@ -23,7 +23,7 @@ __all__ = [
     "Plan", "Cache", "BadConfiguration", "BbfftError", "DescriptorError", "kernel_configuration",
     "generate_source", "aot_generate", "aot_generate_shard", "device_info_query", "MAX_FFT_DIM",
     "MAX_TENSOR_DIM", "default_cache", "root_of_unity", "factor", "num_passes", "compile_kernel",
-    "normalize_device_info", "GFX950_INFO",
+    "normalize_device_info", "GFX950_INFO", "adjoint_configuration", "make_adjoint_plan",
 ]
 
 MAX_FFT_DIM = 3
@@ -385,6 +385,28 @@ class Plan:
         if lib is not None and getattr(self, "_h", None):
             lib.bbfft_plan_destroy(self._h)
             self._h = None
+
+
+def adjoint_configuration(cfg: configuration) -> configuration:
+    """Configuration of the adjoint transform of cfg (see bbfft_adjoint_configuration)."""
+    c = _capi.CConfiguration()
+    _check(_capi.load().bbfft_adjoint_configuration(ctypes.byref(cfg._to_c()), ctypes.byref(c)))
+    return configuration(dim=c.dim, shape=list(c.shape), fp=precision(c.precision),
+                         dir=direction(c.direction), type=transform_type(c.type),
+                         istride=list(c.istride), ostride=list(c.ostride))
+
+
+def make_adjoint_plan(cfg, stream=None, cache: Optional["Cache"] = None) -> "Plan":
+    """Plan computing the adjoint of cfg's transform (gradient of a real loss)."""
+    if isinstance(cfg, str):
+        cfg = parse_fft_descriptor(cfg)
+    if stream is None:
+        stream = _current_stream()
+    h = ctypes.c_void_p()
+    _check(_capi.load().bbfft_plan_create_adjoint(ctypes.byref(cfg._to_c()), stream,
+                                                  cache.handle if cache is not None else None,
+                                                  ctypes.byref(h)))
+    return Plan(h, adjoint_configuration(cfg), stream)
 
 
 DEFAULT_AOT_DIR = os.path.join(os.path.dirname(_capi.DEFAULT_LIB), "aot")

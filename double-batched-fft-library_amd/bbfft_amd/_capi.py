@@ -18,7 +18,7 @@ EXPORTED = [
     "bbfft_root_of_unity",
     "bbfft_num_passes", "bbfft_kernel_configuration", "bbfft_generate_source",
     "bbfft_compile_kernel", "bbfft_kernel_configuration_tuned", "bbfft_generate_source_tuned",
-    "bbfft_plan_create_tuned",
+    "bbfft_plan_create_tuned", "bbfft_adjoint_configuration", "bbfft_plan_create_adjoint",
     "bbfft_cache_create", "bbfft_cache_load_aot", "bbfft_cache_size", "bbfft_cache_destroy",
     "bbfft_plan_create", "bbfft_plan_create_from_descriptor", "bbfft_plan_execute",
     "bbfft_plan_num_launches", "bbfft_plan_launch_info", "bbfft_plan_destroy",
@@ -119,6 +119,9 @@ def load():
                                                 c_char_p, c_size, P(c_size)]),
         "bbfft_plan_create_tuned": (c_int, [P(CConfiguration), P(CTuning), c_void_p, c_void_p,
                                             P(c_void_p)]),
+        "bbfft_adjoint_configuration": (c_int, [P(CConfiguration), P(CConfiguration)]),
+        "bbfft_plan_create_adjoint": (c_int, [P(CConfiguration), c_void_p, c_void_p,
+                                              P(c_void_p)]),
         "bbfft_cache_create": (c_int, [P(c_void_p)]),
         "bbfft_cache_load_aot": (c_int, [c_void_p, c_char_p, c_int, P(c_size)]),
         "bbfft_cache_size": (c_int, [c_void_p, P(c_size)]),

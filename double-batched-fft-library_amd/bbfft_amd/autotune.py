@@ -1,0 +1,243 @@
+"""Autotuner for the gfx950 kernels: picks the radix factorisation, the work-items per
+transform, the M x K tile per work-group, the LDS padding and the streaming (non-temporal)
+hints for each (precision, transform type, N, M), and writes the table the library compiles in
+(double-batched-fft-library_amd/tuning/gfx950.inc).
+
+    python -m bbfft_amd.autotune [-o TABLE] [--rounds 2] DESCRIPTOR...
+
+Search: coordinate descent from the library's heuristic default, one parameter group at a time
+(factorisation + work-items, K tile, M tile, LDS padding, flags), keeping the fastest. Every
+candidate's output is compared with the default plan's output on the same input (relative L2
+within the precision's tolerance) before it is timed; the default plan itself is covered by the
+GPU parity suite against the FP64 oracle. Time = best of 3 runs of `reps` launches (HIP events).
+"""
+import argparse
+import itertools
+import json
+import math
+import os
+import sys
+import time
+
+from . import (GFX950_INFO, BadConfiguration, BbfftError, default_cache, kernel_configuration,
+               make_plan, parse_fft_descriptor)
+
+TABLE_HEADER = ("# Tuned kernel parameters for gfx950 (MI355X), written by "
+                "`python -m bbfft_amd.autotune`.\n"
+                "# precision type N M : factorization Nt Mb Kb lds_pad flags"
+                "   # GB/s default -> tuned, descriptor\n")
+
+
+def _divisors(n):
+    return [d for d in range(1, n + 1) if n % d == 0]
+
+
+def candidate_factorizations(nfft, fp, limit=8):
+    """Ordered factorisations into 1-3 radices in [2, 32], fewest stages / most balanced first."""
+    rmax = 32
+    cands = set()
+    if 2 <= nfft <= rmax:
+        cands.add((nfft,))
+    for a in _divisors(nfft):
+        b = nfft // a
+        if 2 <= a <= rmax and 2 <= b <= rmax:
+            cands.add((a, b))
+        for c in _divisors(b):
+            d = b // c
+            if 2 <= a <= rmax and 2 <= c <= rmax and 2 <= d <= rmax:
+                cands.add((a, c, d))
+    if nfft == 1:
+        cands.add((1,))
+
+    def score(f):
+        return (len(f), max(f) - min(f), -f[0])
+
+    return [list(f) for f in sorted(cands, key=score)[:limit]]
+
+
+def _pow2_upto(n):
+    v, out = 1, []
+    while v <= n:
+        out.append(v)
+        v *= 2
+    return out
+
+
+class Tuner:
+    def __init__(self, desc, reps=10, verbose=True):
+        import torch
+        self.torch = torch
+        self.desc = desc
+        self.cfg = parse_fft_descriptor(desc)
+        if self.cfg.dim != 1:
+            raise ValueError("autotuning applies to 1-D descriptors")
+        self.reps = reps
+        self.verbose = verbose
+        self.cache = default_cache()
+        M, N, K = self.cfg.shape[:3]
+        self.M, self.N, self.K = M, N, K
+        self.fp = int(self.cfg.fp)
+        t = int(self.cfg.type)
+        self.type = t
+        real = t != 0
+        self.nfft = N // 2 if real and N % 2 == 0 else N
+        rdt = torch.float32 if self.fp == 4 else torch.float64
+        g = torch.Generator(device="cuda")
+        g.manual_seed(N * 7 + M)
+        istride, ostride = self.cfg.istride, self.cfg.ostride
+        ishape = [M, N if t != 2 else N // 2 + 1, K]
+        oshape = [M, N if t != 1 else N // 2 + 1, K]
+        n_in = sum((ishape[i] - 1) * istride[i] for i in range(3)) + 1
+        n_out = sum((oshape[i] - 1) * ostride[i] for i in range(3)) + 1
+        in_elems = n_in if t == 1 else 2 * n_in
+        out_elems = n_out if t == 2 else 2 * n_out
+        self.x = torch.rand(in_elems, generator=g, device="cuda", dtype=rdt) * 2 - 1
+        self.y = torch.empty(out_elems, device="cuda", dtype=rdt)
+        self.ref = torch.empty_like(self.y)
+        self.bytes = (in_elems + out_elems) * (4 if self.fp == 4 else 8)
+        self.default_ms = None
+        self.results = {}
+
+    def log(self, **kw):
+        if self.verbose:
+            print(json.dumps(kw), flush=True)
+
+    def time_plan(self, plan):
+        torch = self.torch
+        plan.execute(self.x, self.y)
+        plan.execute(self.x, self.y)
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(3):
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record()
+            for _ in range(self.reps):
+                plan.execute(self.x, self.y)
+            e.record()
+            torch.cuda.synchronize()
+            best = min(best, s.elapsed_time(e) / self.reps)
+        return best
+
+    def evaluate(self, tuning):
+        key = json.dumps(tuning, sort_keys=True)
+        if key in self.results:
+            return self.results[key]
+        torch = self.torch
+        try:
+            plan = make_plan(self.cfg, cache=self.cache, tuning=tuning)
+            self.y.zero_()
+            plan.execute(self.x, self.y)
+            torch.cuda.synchronize()
+            err = (torch.linalg.vector_norm(self.y - self.ref) /
+                   torch.linalg.vector_norm(self.ref)).item()
+            tol = 8.0 * (1.2e-7 if self.fp == 4 else 2.3e-16) * max(1.0, math.log2(self.N))
+            if not err <= tol:
+                self.log(desc=self.desc, tuning=tuning, rejected=f"rel err {err:.3g} > {tol:.3g}")
+                ms = float("inf")
+            else:
+                ms = self.time_plan(plan)
+        except (BadConfiguration, BbfftError, ValueError) as ex:
+            self.log(desc=self.desc, tuning=tuning, skipped=str(ex).splitlines()[0][:160])
+            ms = float("inf")
+        self.results[key] = ms
+        if ms < float("inf"):
+            self.log(desc=self.desc, tuning=tuning, ms=round(ms, 5),
+                     GBps=round(self.bytes / ms / 1e6, 1))
+        return ms
+
+    def full_tuning(self, partial):
+        """Complete a partial tuning with the values the library would choose for it."""
+        kc = kernel_configuration(self.cfg, tuning=partial, device_info=GFX950_INFO)
+        return {"factorization": kc["factorization"], "Nt": kc["Nt"], "Mb": kc["Mb"],
+                "Kb": kc["Kb"], "lds_pad": kc["lds_pad"], "no_pad": kc["lds_pad"] == 0,
+                "flags": partial.get("flags", 0)}
+
+    def run(self, rounds=2):
+        torch = self.torch
+        # reference output and time of the heuristic default (explicit, table-independent)
+        base = self.full_tuning({"flags": 0})
+        plan = make_plan(self.cfg, cache=self.cache, tuning=base)
+        plan.execute(self.x, self.ref)
+        torch.cuda.synchronize()
+        self.default_ms = self.time_plan(plan)
+        self.results[json.dumps(base, sort_keys=True)] = self.default_ms
+        self.log(desc=self.desc, default=base, ms=round(self.default_ms, 5),
+                 GBps=round(self.bytes / self.default_ms / 1e6, 1))
+        best, best_ms = dict(base), self.default_ms
+        kcols = (self.K + 1) // 2 if (self.type != 0 and self.N % 2 == 1) else self.K
+        for _ in range(rounds):
+            # 1) factorisation and work-items per transform (tile re-derived by the library)
+            opts = []
+            for f in candidate_factorizations(self.nfft, self.fp):
+                nt0 = self.nfft // max(f)
+                for nt in sorted({nt0, 2 * nt0}):
+                    if nt <= 256:
+                        opts.append({"factorization": f, "Nt": nt, "flags": best["flags"],
+                                     "lds_pad": best["lds_pad"], "no_pad": best["no_pad"]})
+            best, best_ms = self._try(best, best_ms, opts, complete=True)
+            # 2) K tile
+            opts = [dict(best, Kb=kb) for kb in _pow2_upto(min(64, kcols))
+                    if best["Mb"] * best["Nt"] * kb <= 1024]
+            best, best_ms = self._try(best, best_ms, opts)
+            # 3) M tile
+            mbs = {self.M} | {max(1, -(-self.M // d)) for d in (2, 4)}
+            opts = [dict(best, Mb=mb) for mb in sorted(mbs) if mb * best["Nt"] <= 1024]
+            best, best_ms = self._try(best, best_ms, opts)
+            # 4) LDS padding
+            opts = [dict(best, lds_pad=p, no_pad=p == 0) for p in (0, 8, 16, 32)]
+            best, best_ms = self._try(best, best_ms, opts)
+            # 5) streaming hints
+            opts = [dict(best, flags=fl) for fl in (0, 1, 2, 3)]
+            best, best_ms = self._try(best, best_ms, opts)
+        return best, best_ms
+
+    def _try(self, best, best_ms, options, complete=False):
+        for o in options:
+            t = self.full_tuning(o) if complete else o
+            t["flags"] = o.get("flags", 0)
+            ms = self.evaluate(t)
+            if ms < best_ms * 0.995:  # keep the incumbent unless clearly faster
+                best, best_ms = t, ms
+        return best, best_ms
+
+
+def table_line(cfg, tuning, default_ms, best_ms, nbytes, desc):
+    prec = "f32" if int(cfg.fp) == 4 else "f64"
+    typ = ["c2c", "r2c", "c2r"][int(cfg.type)]
+    fac = "x".join(str(f) for f in tuning["factorization"])
+    return (f"{prec} {typ} {cfg.shape[1]} {cfg.shape[0]} : {fac} {tuning['Nt']} {tuning['Mb']} "
+            f"{tuning['Kb']} {tuning['lds_pad']} {tuning['flags']}   # "
+            f"{nbytes / default_ms / 1e6:.0f} -> {nbytes / best_ms / 1e6:.0f} GB/s, {desc}")
+
+
+def write_table(path, lines):
+    with open(path, "w") as f:
+        f.write('R"TABLE(\n' + TABLE_HEADER + "\n".join(lines) + '\n)TABLE"\n')
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("-o", "--output", default=None, help="table file to write (.inc)")
+    ap.add_argument("--rounds", type=int, default=2)
+    ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("descriptors", nargs="+")
+    a = ap.parse_args(argv)
+    lines = []
+    for desc in a.descriptors:
+        t0 = time.time()
+        tuner = Tuner(desc, reps=a.reps)
+        best, best_ms = tuner.run(a.rounds)
+        line = table_line(tuner.cfg, best, tuner.default_ms, best_ms, tuner.bytes, desc)
+        lines.append(line)
+        print(json.dumps({"desc": desc, "best": best, "ms": round(best_ms, 5),
+                          "default_ms": round(tuner.default_ms, 5),
+                          "speedup": round(tuner.default_ms / best_ms, 4),
+                          "seconds": round(time.time() - t0, 1), "line": line}), flush=True)
+        if a.output:
+            write_table(a.output, lines)
+        del tuner
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -283,6 +283,24 @@ int bbfft_generate_source_tuned(const bbfft_configuration *cfg, const bbfft_tuni
     });
 }
 
+int bbfft_adjoint_configuration(const bbfft_configuration *cfg, bbfft_configuration *adjoint) {
+    return guarded([&] {
+        require(cfg && adjoint, "null argument");
+        unsigned variant = 0;
+        to_c(adjoint_configuration(from_c(*cfg), &variant), *adjoint);
+    });
+}
+
+int bbfft_plan_create_adjoint(const bbfft_configuration *cfg, bbfft_stream_t stream,
+                              bbfft_cache_t cache, bbfft_plan_t *plan) {
+    return guarded([&] {
+        require(cfg && plan, "null argument");
+        auto impl = hip_detail::create_adjoint(from_c(*cfg), static_cast<hipStream_t>(stream),
+                                               cache);
+        *plan = new bbfft_plan_s{std::move(impl)};
+    });
+}
+
 int bbfft_plan_create_tuned(const bbfft_configuration *cfg, const bbfft_tuning *tuning,
                             bbfft_stream_t stream, bbfft_cache_t cache, bbfft_plan_t *plan) {
     return guarded([&] {

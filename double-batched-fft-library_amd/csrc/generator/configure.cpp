@@ -5,6 +5,7 @@
 #include "../base/math.hpp"
 #include "bbfft/bad_configuration.hpp"
 #include "bbfft/detail/generator_impl.hpp"
+#include "tuning_table.hpp"
 
 #include <algorithm>
 #include <numeric>
@@ -94,7 +95,7 @@ std::vector<int> default_factorization(std::size_t n, std::size_t single_max,
 } // namespace
 
 fft1d_configuration configure_fft1d(configuration const &cfg, device_info const &info,
-                                    fft1d_tuning const &tuning) {
+                                    fft1d_tuning const &tuning_in, unsigned variant) {
     if (cfg.dim != 1) {
         throw bad_configuration("configure_fft1d requires a 1-D configuration");
     }
@@ -108,6 +109,15 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     std::size_t const M = cfg.shape[0], N = cfg.shape[1], K = cfg.shape[2];
     if (M == 0 || N == 0 || K == 0) {
         throw bad_configuration("Tensor shape must not contain zeros");
+    }
+    // Without explicit tuning, a measured entry of the persisted table (if any) is used; its
+    // K tile is clamped to the number of K columns below.
+    fft1d_tuning tuning = tuning_in;
+    bool const explicit_tuning = !tuning_in.factorization.empty() || tuning_in.Nt ||
+                                 tuning_in.Mb || tuning_in.Kb || tuning_in.lds_pad ||
+                                 tuning_in.no_pad || tuning_in.has_flags;
+    if (!explicit_tuning && info.type == device_type::gpu && info.max_subgroup_size() == 64) {
+        gen::lookup_tuning(cfg.fp, cfg.type, N, M, tuning);
     }
 
     fft1d_configuration c = {};
@@ -187,7 +197,7 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
         }
     }
     if (tuning.Kb) {
-        c.Kb = tuning.Kb;
+        c.Kb = std::min(tuning.Kb, kcols);
     } else {
         std::size_t target = 256;
         std::size_t kb = std::max<std::size_t>(1, target / (c.Mb * c.Nt));
@@ -203,7 +213,10 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     if (c.lds_bytes() > info.local_memory_size) {
         throw bad_configuration("Work-group exceeds the local memory size");
     }
-    c.flags = tuning.has_flags ? tuning.flags : 0u;
+    c.flags = (tuning.has_flags ? tuning.flags : 0u) | variant;
+    if ((c.flags & fft1d_adjoint) && cfg.type == transform_type::c2c) {
+        throw bad_configuration("The adjoint variant applies to real transforms only");
+    }
     c.inplace_unsupported = cfg.type != transform_type::c2c && c.Mb < M;
     return c;
 }

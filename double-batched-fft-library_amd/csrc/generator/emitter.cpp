@@ -602,6 +602,12 @@ class kernel_gen {
                 w_.line(T2() + " x1 = " + gload(ioff(q)) + ";");
                 w_.line("if (" + q + " == 0u) x1.y = " + f_.lit(0.0) + ";");
                 w_.line(T2() + " x2 = " + gload(ioff(u(H) + " - " + q)) + ";");
+                if (adjoint()) {
+                    // Hermitian fold: halve the non-self-conjugate bins, drop imag(Nyquist)
+                    w_.line("if (" + q + " != 0u) { x1 = c_scale(x1, " + f_.lit(0.5) +
+                            "); x2 = c_scale(x2, " + f_.lit(0.5) + "); } else { x2.y = " +
+                            f_.lit(0.0) + "; }");
+                }
                 w_.line("x2.y = -x2.y;");
                 w_.line("const " + T2() + " a = c_add(x1, x2);");
                 w_.line("const " + T2() + " bt = c_mul(c_sub(x1, x2), " + twr_name() + "[" + q +
@@ -615,6 +621,10 @@ class kernel_gen {
                 w_.line("if (active_b) bi = " + gload(ioff(q, 1)) + ";");
                 w_.line("if (" + q + " == 0u) { ai.y = " + f_.lit(0.0) + "; bi.y = " +
                         f_.lit(0.0) + "; }");
+                if (adjoint()) {
+                    w_.line("if (" + q + " != 0u) { ai = c_scale(ai, " + f_.lit(0.5) +
+                            "); bi = c_scale(bi, " + f_.lit(0.5) + "); }");
+                }
                 w_.line("const " + T2() + " ib2 = c_ipos(bi);");
                 w_.line(lds(q) + " = c_add(ai, ib2);");
                 w_.line("if (" + q + " != 0u) { const " + T2() + " d = c_sub(ai, ib2); " +
@@ -643,21 +653,43 @@ class kernel_gen {
                 w_.line("const " + T2() + " a = c_scale(c_add(y2, y1), " + half + ");");
                 w_.line("const " + T2() + " bt = c_mul(c_scale(c_sub(y2, y1), " + half + "), " +
                         twr_name() + "[" + q + "]);");
-                w_.line(gstore(ooff(q), "c_add(a, bt)"));
-                w_.line("if (2u * " + q + " != " + u(H) + ") " +
-                        gstore(ooff(u(H) + " - " + q), "mk(a.x - bt.x, bt.y - a.y)"));
+                if (adjoint()) {
+                    // doubled non-self-conjugate bins; imag(X[N/2]) = -real(X[0])
+                    std::string two = f_.lit(2.0);
+                    w_.line("const " + T2() + " z0 = c_add(a, bt);");
+                    w_.line("const " + T2() + " z1 = mk(a.x - bt.x, bt.y - a.y);");
+                    w_.open("if (" + q + " == 0u)");
+                    w_.line(gstore(ooff(q), "z0"));
+                    w_.line(gstore(ooff(u(H)), "mk(z1.x, -z0.x)"));
+                    w_.close();
+                    w_.open("else");
+                    w_.line(gstore(ooff(q), "c_scale(z0, " + two + ")"));
+                    w_.line("if (2u * " + q + " != " + u(H) + ") " +
+                            gstore(ooff(u(H) + " - " + q), "c_scale(z1, " + two + ")"));
+                    w_.close();
+                } else {
+                    w_.line(gstore(ooff(q), "c_add(a, bt)"));
+                    w_.line("if (2u * " + q + " != " + u(H) + ") " +
+                            gstore(ooff(u(H) + " - " + q), "mk(a.x - bt.x, bt.y - a.y)"));
+                }
             } else {
                 w_.line("const " + T2() + " y1 = " + lds(q) + ";");
                 w_.line(T2() + " y2 = " + lds("(" + u(c_.N) + " - " + q + ") % " + u(c_.N)) +
                         ";");
                 w_.line("y2.y = -y2.y;");
-                w_.line(gstore(ooff(q), "c_scale(c_add(y2, y1), " + half + ")"));
-                w_.line("if (active_b) { const " + T2() + " d = c_scale(c_sub(y2, y1), " + half +
+                // adjoint: doubled non-self-conjugate bins (q >= 1), i.e. no 1/2 there
+                std::string w = adjoint() ? "(" + q + " == 0u ? " + half + " : " +
+                                                f_.lit(1.0) + ")"
+                                          : half;
+                w_.line(gstore(ooff(q), "c_scale(c_add(y2, y1), " + w + ")"));
+                w_.line("if (active_b) { const " + T2() + " d = c_scale(c_sub(y2, y1), " + w +
                         "); " + gstore(ooff(q, 1), "mk(-d.y, d.x)") + " }");
             }
             w_.close();
         }
     }
+
+    bool adjoint() const { return (c_.flags & fft1d_adjoint) != 0; }
 
     fft1d_configuration const &c_;
     fp_helper f_;

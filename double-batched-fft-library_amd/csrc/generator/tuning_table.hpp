@@ -1,0 +1,19 @@
+// Lookup into the persisted autotuning table.
+#ifndef BBFFT_AMD_TUNING_TABLE_HPP
+#define BBFFT_AMD_TUNING_TABLE_HPP
+
+#include "bbfft/configuration.hpp"
+#include "bbfft/detail/generator_impl.hpp"
+
+#include <cstddef>
+
+namespace bbfft::gen {
+
+/// Tuned parameters for (precision, type, N, M), if the table has an entry.
+bool lookup_tuning(precision fp, transform_type type, std::size_t N, std::size_t M,
+                   fft1d_tuning &out);
+std::size_t tuning_table_size();
+
+} // namespace bbfft::gen
+
+#endif // BBFFT_AMD_TUNING_TABLE_HPP

@@ -34,7 +34,7 @@ class plan_base : public detail::plan_unmanaged_event_impl<hipEvent_t> {
 class fft1d_plan final : public plan_base {
   public:
     fft1d_plan(configuration const &cfg, hipStream_t stream, int device, jit_cache *cache,
-               fft1d_tuning const &tuning = {});
+               fft1d_tuning const &tuning = {}, unsigned variant = 0u);
     void execute(void const *in, void *out, hipEvent_t signal, std::uint32_t nwait,
                  hipEvent_t *wait) override;
     /// Launch on the plan's stream without event handling.
@@ -57,7 +57,8 @@ class fft1d_plan final : public plan_base {
 /// 2-D/3-D transform as a chain of 1-D passes on one stream.
 class nd_plan final : public plan_base {
   public:
-    nd_plan(pass_layout const &layout, hipStream_t stream, int device, jit_cache *cache);
+    nd_plan(pass_layout const &layout, hipStream_t stream, int device, jit_cache *cache,
+            unsigned real_pass_variant = 0u);
     ~nd_plan();
     void execute(void const *in, void *out, hipEvent_t signal, std::uint32_t nwait,
                  hipEvent_t *wait) override;
@@ -71,6 +72,9 @@ class nd_plan final : public plan_base {
 
 std::shared_ptr<plan_base> create(configuration const &cfg, hipStream_t stream,
                                   jit_cache *cache);
+/// Plan of the adjoint of cfg's transform (see adjoint_configuration).
+std::shared_ptr<plan_base> create_adjoint(configuration const &cfg, hipStream_t stream,
+                                          jit_cache *cache);
 /// 1-D plan with explicit tuning.
 std::shared_ptr<plan_base> create_tuned(configuration const &cfg, fft1d_tuning const &tuning,
                                         hipStream_t stream, jit_cache *cache);

@@ -32,9 +32,9 @@ void release_module(module_handle_t m) { (void)hipModuleUnload(reinterpret_cast<
 namespace hip_detail {
 
 fft1d_plan::fft1d_plan(configuration const &cfg, hipStream_t stream, int device,
-                       jit_cache *cache, fft1d_tuning const &tuning)
+                       jit_cache *cache, fft1d_tuning const &tuning, unsigned variant)
     : stream_(stream), device_(device) {
-    kc_ = configure_fft1d(cfg, hip::get_device_info(device), tuning);
+    kc_ = configure_fft1d(cfg, hip::get_device_info(device), tuning, variant);
     name_ = kc_.identifier();
     K_ = cfg.shape[2];
     jit_cache_key key{name_, hip::get_device_id(device)};
@@ -103,10 +103,13 @@ std::vector<launch_info> fft1d_plan::launches() const {
     return {launch_info{name_, grid_, block_, kc_.lds_bytes()}};
 }
 
-nd_plan::nd_plan(pass_layout const &layout, hipStream_t stream, int device, jit_cache *cache)
+nd_plan::nd_plan(pass_layout const &layout, hipStream_t stream, int device, jit_cache *cache,
+                 unsigned real_pass_variant)
     : stream_(stream) {
     for (auto const &p : layout.passes) {
-        passes_.push_back(std::make_unique<fft1d_plan>(p, stream, device, cache));
+        unsigned v = p.type == transform_type::c2c ? 0u : real_pass_variant;
+        passes_.push_back(std::make_unique<fft1d_plan>(p, stream, device, cache,
+                                                       fft1d_tuning{}, v));
     }
     if (layout.tmp_bytes) {
         BBFFT_HIP_CHECK(hipMalloc(&tmp_, layout.tmp_bytes));
@@ -159,6 +162,26 @@ std::shared_ptr<plan_base> create(configuration const &cfg, hipStream_t stream,
         return std::make_shared<fft1d_plan>(layout.passes[0], stream, device, cache);
     }
     return std::make_shared<nd_plan>(layout, stream, device, cache);
+}
+
+std::shared_ptr<plan_base> create_adjoint(configuration const &cfg, hipStream_t stream,
+                                          jit_cache *cache) {
+    if (cfg.dim < 1 || cfg.dim > max_fft_dim) {
+        throw bad_configuration("Unsupported FFT dimension: " + std::to_string(cfg.dim));
+    }
+    if (cfg.callbacks) {
+        throw bad_configuration("Adjoint plans do not support callbacks");
+    }
+    unsigned variant = 0;
+    auto adj = adjoint_configuration(cfg, &variant);
+    auto layout = decompose(adj);
+    int device = 0;
+    BBFFT_HIP_CHECK(hipGetDevice(&device));
+    if (layout.passes.size() == 1) {
+        return std::make_shared<fft1d_plan>(layout.passes[0], stream, device, cache,
+                                            fft1d_tuning{}, variant);
+    }
+    return std::make_shared<nd_plan>(layout, stream, device, cache, variant);
 }
 
 std::shared_ptr<plan_base> create_tuned(configuration const &cfg, fft1d_tuning const &tuning,

@@ -1,5 +1,6 @@
 #include "decompose.hpp"
 #include "bbfft/bad_configuration.hpp"
+#include "bbfft/detail/generator_impl.hpp"
 
 #include <string>
 #include <utility>
@@ -100,6 +101,29 @@ pass_layout decompose(configuration const &cfg) {
         out.tmp_bytes = isize;
     }
     return out;
+}
+
+configuration adjoint_configuration(configuration const &cfg, unsigned *real_pass_variant) {
+    configuration adj = cfg;
+    adj.callbacks = {};
+    std::swap(adj.istride, adj.ostride);
+    switch (cfg.type) {
+    case transform_type::c2c:
+        adj.dir = cfg.dir == direction::forward ? direction::backward : direction::forward;
+        *real_pass_variant = 0;
+        break;
+    case transform_type::r2c:
+        adj.type = transform_type::c2r;
+        adj.dir = direction::backward;
+        *real_pass_variant = fft1d_adjoint;
+        break;
+    case transform_type::c2r:
+        adj.type = transform_type::r2c;
+        adj.dir = direction::forward;
+        *real_pass_variant = fft1d_adjoint;
+        break;
+    }
+    return adj;
 }
 
 } // namespace bbfft

@@ -24,6 +24,16 @@ struct pass_layout {
  */
 pass_layout decompose(configuration const &cfg);
 
+/**
+ * @brief Configuration of the adjoint transform (transpose w.r.t. the real inner product).
+ *
+ * c2c: the opposite direction with input and output strides swapped (the unnormalised DFT
+ * matrix is symmetric, so its adjoint is its conjugate). r2c (c2r): a c2r (r2c) transform from
+ * the output layout to the input layout; its real pass must run with the fft1d_adjoint variant
+ * (Hermitian-fold weighting), which `real_pass_variant` reports.
+ */
+configuration adjoint_configuration(configuration const &cfg, unsigned *real_pass_variant);
+
 } // namespace bbfft
 
 #endif // BBFFT_AMD_DECOMPOSE_HPP

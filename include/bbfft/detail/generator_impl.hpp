@@ -62,6 +62,13 @@ struct BBFFT_EXPORT fft1d_configuration {
 enum fft1d_flag : unsigned {
     fft1d_nontemporal_load = 1u,  ///< stream the input past the caches (read once)
     fft1d_nontemporal_store = 2u, ///< stream the output past the caches (written once)
+    /// Real transforms only: the kernel computes the adjoint (transpose w.r.t. the real inner
+    /// product) of the opposite real transform. c2r: the Hermitian-fold weights (1/2 on the
+    /// non-self-conjugate bins, Nyquist imaginary part dropped) are applied in the pre-pass,
+    /// making it the adjoint of r2c. r2c: non-self-conjugate bins are doubled and, for even N,
+    /// imag(X[N/2]) = -real(X[0]) in the post-pass, making it the adjoint of c2r (including the
+    /// c2r Nyquist rule).
+    fft1d_adjoint = 4u,
 };
 
 /// Optional overrides of the tiling heuristics (0 / empty = choose automatically).
@@ -76,10 +83,12 @@ struct BBFFT_EXPORT fft1d_tuning {
     bool has_flags = false; ///< use `flags` instead of the default
 };
 
-/// Choose a kernel configuration for a 1-D configuration (dim == 1).
+/// Choose a kernel configuration for a 1-D configuration (dim == 1). `variant` (fft1d_flag
+/// bits such as fft1d_adjoint) is added to the flags whatever the tuning says.
 BBFFT_EXPORT fft1d_configuration configure_fft1d(configuration const &cfg,
                                                  device_info const &info,
-                                                 fft1d_tuning const &tuning = {});
+                                                 fft1d_tuning const &tuning = {},
+                                                 unsigned variant = 0u);
 /**
  * @brief Emit HIP source (no includes; hiprtc provides the HIP device builtins) for one kernel.
  *

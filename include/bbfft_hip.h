@@ -150,6 +150,16 @@ BBFFT_EXPORT_C int bbfft_plan_create(const bbfft_configuration *cfg, bbfft_strea
                                      bbfft_cache_t cache, bbfft_plan_t *plan);
 BBFFT_EXPORT_C int bbfft_plan_create_from_descriptor(const char *desc, bbfft_stream_t stream,
                                                      bbfft_cache_t cache, bbfft_plan_t *plan);
+/* Adjoint (transpose w.r.t. the real inner product) of cfg's transform, for differentiation.
+ * No reference counterpart (an extension): c2c -> opposite-sign unnormalised transform with
+ * in/out layouts swapped; r2c -> Hermitian-fold weighting + c2r; c2r -> r2c + doubling of the
+ * non-self-conjugate bins. bbfft_adjoint_configuration returns the adjoint's configuration
+ * (its real pass additionally runs the weighting variant). */
+BBFFT_EXPORT_C int bbfft_adjoint_configuration(const bbfft_configuration *cfg,
+                                               bbfft_configuration *adjoint);
+BBFFT_EXPORT_C int bbfft_plan_create_adjoint(const bbfft_configuration *cfg,
+                                             bbfft_stream_t stream, bbfft_cache_t cache,
+                                             bbfft_plan_t *plan);
 /* make_plan of a 1-D configuration with explicit tuning (autotuner candidates). */
 BBFFT_EXPORT_C int bbfft_plan_create_tuned(const bbfft_configuration *cfg,
                                            const bbfft_tuning *tuning, bbfft_stream_t stream,

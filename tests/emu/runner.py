@@ -40,14 +40,14 @@ def compile_kernel(source, kernel_name, in_type, out_type):
 
 
 def run(bbfft, cfg, x, out_size, grid_cap=None, device_info=None, pass_index=0,
-        in_place_buffer=None):
+        in_place_buffer=None, tuning=None):
     """Execute 1-D pass `pass_index` of cfg on host arrays through the emulator.
 
     x: input array (complex or real per the pass); returns out_size output elements. With
     in_place_buffer the kernel reads and writes that buffer (in == out)."""
     info = device_info or bbfft.GFX950_INFO
-    kc = bbfft.kernel_configuration(cfg, pass_index, info)
-    src = bbfft.generate_source(cfg, pass_index, info)
+    kc = bbfft.kernel_configuration(cfg, pass_index, info, tuning=tuning)
+    src = bbfft.generate_source(cfg, pass_index, info, tuning=tuning)
     f32 = int(cfg.fp) == 4
     real_t, cplx_t = ("float", "float2") if f32 else ("double", "double2")
     npc, npr = (np.complex64, np.float32) if f32 else (np.complex128, np.float64)

@@ -133,3 +133,31 @@ def test_callbacks_bitwise(fp, M, N):
         y = runner.run(bb, cfg, x, M * n_h * K)[:M * n_cut * K].reshape(K, n_cut, M)
         assert np.array_equal(y.real, ref.real[:, :n_cut] * rt(scale))
         assert np.array_equal(y.imag, ref.imag[:, :n_cut] * rt(scale))
+
+
+def _real_inner(a, b):
+    return float(np.vdot(np.asarray(a, np.complex128).ravel(),
+                         np.asarray(b, np.complex128).ravel()).real)
+
+
+@pytest.mark.parametrize("N", [8, 9, 16, 15, 60, 105, 256])
+@pytest.mark.parametrize("t", [0, 1, 2])
+def test_adjoint_kernels_inner_product_identity(N, t):
+    """<A x, y> = <x, A^T y> (real inner product) for the forward kernel A and the adjoint
+    kernel (opposite-sign c2c; Hermitian-fold c2r for r2c; doubling r2c for c2r)."""
+    M, K, fp = 3, 5, 8
+    d = 1 if t == 2 else -1
+    cfg = _cfg(M, N, K, fp, d, t, False)
+    adj = bb.adjoint_configuration(cfg)
+    H = N // 2 + 1
+    ishape = [M, H if t == 2 else N, K]
+    oshape = [M, H if t == 1 else N, K]
+    n_in = oracle.extent(1, ishape, cfg.istride)
+    n_out = oracle.extent(1, oshape, cfg.ostride)
+    x = oracle.uniform_real(1, n_in) if t == 1 else oracle.uniform_complex(1, n_in)
+    y = oracle.uniform_real(2, n_out) if t == 2 else oracle.uniform_complex(2, n_out)
+    Ax = runner.run(bb, cfg, x, n_out)
+    variant = {"flags": 4} if t != 0 else {"flags": 0}
+    ATy = runner.run(bb, adj, y, n_in, tuning=variant)
+    lhs, rhs = _real_inner(Ax, y), _real_inner(x, ATy)
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + np.linalg.norm(Ax) * np.linalg.norm(y))

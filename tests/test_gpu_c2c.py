@@ -140,7 +140,10 @@ def test_plan_reuse_and_events():
 
 
 def test_launch_introspection():
-    plan = bb.make_plan("scfo8.256*32768", cache=bb.default_cache())
+    cfg = bb.parse_fft_descriptor("scfo8.256*32768")
+    kc = bb.kernel_configuration(cfg)
+    plan = bb.make_plan(cfg, cache=bb.default_cache())
     (l,) = plan.launches()
-    assert l["kernel"].startswith("bbfft_m1_M8_Mb8_N256")
-    assert l["block"] == 256 and l["grid"] == 16384
+    assert l["kernel"] == kc["name"] and l["kernel"].startswith("bbfft_m1_M8_Mb8_N256")
+    assert l["block"] == kc["work_group_size"]
+    assert l["grid"] == 32768 // kc["Kb"]

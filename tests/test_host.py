@@ -230,19 +230,50 @@ def test_unit_stride_m_required():
 
 
 @pytest.mark.parametrize("desc,fac,Nt,Mb,Kb", [
-    ("scfo8.256*32768", [16, 16], 16, 8, 2),
-    ("scfo8.16*32768", [16], 1, 8, 32),
-    ("scfo8.512*32768", [32, 16], 16, 8, 1),
-    ("dcfo8.512*32768", [8, 8, 8], 64, 4, 1),
-    ("scfo8.60*32768", [10, 6], 6, 8, 5),
-    ("scfo1.64*16384", [8, 8], 8, 1, 32),
+    ("scfo3.256*32768", [16, 16], 16, 3, 5),
+    ("scfo3.16*32768", [16], 1, 3, 85),
+    ("scfo3.512*32768", [32, 16], 16, 3, 5),
+    ("dcfo3.512*32768", [8, 8, 8], 64, 3, 1),
+    ("scfo3.60*32768", [10, 6], 6, 3, 14),
+    ("scfo2.64*16384", [8, 8], 8, 2, 16),
 ])
-def test_kernel_configuration_tiling(desc, fac, Nt, Mb, Kb):
+def test_kernel_configuration_heuristic_tiling(desc, fac, Nt, Mb, Kb):
+    """Heuristics for shapes without a tuned entry (M = 2, 3 are not in the table)."""
     kc = bb.kernel_configuration(bb.parse_fft_descriptor(desc))
     assert (kc["factorization"], kc["Nt"], kc["Mb"], kc["Kb"]) == (fac, Nt, Mb, Kb)
     assert kc["work_group_size"] == Mb * Nt * Kb <= 1024
     assert kc["lds_bytes"] <= 65536
     assert kc["name"].startswith("bbfft_")
+
+
+def _table_entries():
+    path = os.path.join(ROOT, "double-batched-fft-library_amd", "tuning", "gfx950.inc")
+    out = []
+    for line in open(path):
+        line = line.split("#")[0]
+        if ":" not in line:
+            continue
+        lhs, rhs = line.split(":")
+        prec, typ, N, M = lhs.split()
+        fac, Nt, Mb, Kb, pad, flags = rhs.split()
+        out.append((prec, typ, int(N), int(M), [int(f) for f in fac.split("x")], int(Nt),
+                    int(Mb), int(Kb), int(pad), int(flags)))
+    return out
+
+
+def test_tuning_table_is_applied():
+    """Without explicit tuning, configure_fft1d uses the persisted autotuning table."""
+    entries = _table_entries()
+    assert len(entries) >= 20
+    for prec, typ, N, M, fac, Nt, Mb, Kb, pad, flags in entries:
+        d = {"c2c": "c", "r2c": "r", "c2r": "r"}[typ]
+        dirn = "b" if typ == "c2r" else "f"
+        desc = f"{prec[0].replace('f', 's') if prec == 'f32' else 'd'}{d}{dirn}o{M}.{N}*65536"
+        kc = bb.kernel_configuration(bb.parse_fft_descriptor(desc))
+        assert (kc["factorization"], kc["Nt"], kc["Mb"], kc["Kb"], kc["lds_pad"]) == \
+            (fac, Nt, Mb, Kb, pad), desc
+        if flags:
+            assert kc["name"].endswith(f"_g{flags}"), desc
 
 
 def test_nd_decomposition_passes():
