@@ -4,7 +4,8 @@ Workload (headline): plain c2c forward, fp32, M=8 x N=256 x K=32768 (descriptor
 "scfo8.256*32768"), out-of-place, inputs uniform[-1,1) resident in HBM. A step is one
 plan.execute over the whole tensor. Also reported: the sweep N in {16..512, 60, 105, 243, 384}
 for fp32 and fp64 (same M, K), a device-to-device copy bandwidth, and the CPU baseline (the
-oracle's C port of the reference algorithm on a bounded sample, host cores).
+oracle's C port of the reference algorithm on a bounded sample, host cores), and the ragged
+serving workload (4096 mixed jobs in one persistent launch vs per-job plain launches).
 
 Accounting (reference benchmark/adapter.hpp:51-52): flops = 5 N log2 N M K, bytes = 2 M N K
 sizeof(complex). Timing: HIP events on the plan's stream around the timed steps.
@@ -21,6 +22,16 @@ sys.path.insert(0, os.path.join(ROOT, "double-batched-fft-library_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB per dispatch), calibrated
+# against a 1 GiB device copy on MI355X (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000):
+#   rocprofv3 --pmc FETCH_SIZE -- python3 bench.py --no-sweep --no-cpu --steps 3 --warmup 1
+#   rocprofv3 --pmc WRITE_SIZE -- python3 bench.py --no-sweep --no-cpu --steps 3 --warmup 1
+#   rocprofv3 --pmc FETCH_SIZE|WRITE_SIZE -- python3 bench.py --copy-probe
+# Keyed by kernel name; reported only when the timed kernel is the one measured.
+MEASURED_TRAFFIC_BYTES = {
+    "bbfft_m1_M8_Mb8_N256_f8x8x4_Nt32_Kb1_f32_c2c_is1_8_2048_os1_8_2048_p16_in0_g3":
+        (2.0 * 262181.5 + 1.0 * 524288.0) * 1024,
+}
 METRIC = "GFLOP/s (5N·log2N) + %HBM peak; MFMA-split vs plain fp32 at equal FP64-err"
 
 
@@ -67,6 +78,62 @@ def copy_bandwidth(torch, nbytes=1 << 30, reps=10):
     return 2.0 * nbytes * reps / (s.elapsed_time(e) / 1e3) / 1e9
 
 
+def ragged_serving(bb, torch, cache, steps):
+    """The seeded mixed serving workload (4096 c2c fp32 jobs, N in {12, 60, 100, 128, 243, 384,
+    500, 512}, K log-uniform in [64, 65536], M 1..4, alternating directions) in place on one
+    buffer: one persistent launch replayed from a HIP graph, against the same jobs as 4096
+    plain plan launches (each job's default plan) on the same stream."""
+    from bbfft_amd import ragged
+    jobs = ragged.synthetic_serving_jobs()
+    n = ragged.total_elements(jobs)
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info()
+    if free < n * 8 + (8 << 30):
+        return {"skipped": f"needs {n * 8 / 1e9:.0f} GB free device memory"}
+    buf = torch.empty(n, dtype=torch.complex64, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4096)
+
+    def refill():
+        torch.rand(torch.view_as_real(buf).shape, generator=g, device="cuda",
+                   out=torch.view_as_real(buf))
+
+    def timed(fn, reps):
+        refill()
+        fn()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / reps
+
+    nbytes = sum(2.0 * j.elements * 8 for j in jobs)
+    flops = sum(5.0 * j.N * math.log2(j.N) * j.M * j.K for j in jobs)
+    plan = ragged.make_ragged_plan(jobs, 4, cache=cache, graph=True)
+    ms = timed(lambda: plan.execute(buf, buf), max(3, steps // 4))
+    plains = [(bb.make_plan(j.descriptor(), cache=cache), j) for j in jobs]
+
+    def per_job():
+        for p, j in plains:
+            p.execute(buf[j.in_offset:j.in_offset + j.elements])
+
+    ms_plain = timed(per_job, 2)
+    launch = plan.launches()[0]
+    del buf
+    torch.cuda.empty_cache()
+    return {"jobs": len(jobs), "bytes": nbytes, "in_place": True, "graph": True,
+            "ms": round(ms, 4), "GFLOPs": round(flops / ms / 1e6, 1),
+            "GBps": round(nbytes / ms / 1e6, 1), "frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBPS, 4),
+            "per_job_plain_ms": round(ms_plain, 4),
+            "per_job_plain_GBps": round(nbytes / ms_plain / 1e6, 1),
+            "speedup_vs_per_job": round(ms_plain / ms, 3),
+            "kernel": launch["kernel"], "grid": launch["grid"], "block": launch["block"],
+            "lds_bytes": launch["lds_bytes"]}
+
+
 def cpu_baseline(budget_s=15.0):
     """Oracle port of the reference algorithm on host cores, bounded sample of the workload."""
     try:
@@ -83,6 +150,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ragged", action="store_true")
     ap.add_argument("--desc", default="scfo8.256*32768")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (counter calibration)")
@@ -111,6 +179,10 @@ def main():
                               "GFLOPs": round(r["gflops"], 1), "GBps": round(r["GBps"], 1),
                               "frac": round(r["frac"], 4)})
                 print(json.dumps(sweep[-1]), file=sys.stderr, flush=True)
+    rag = None
+    if not a.no_ragged:
+        rag = ragged_serving(bb, torch, cache, a.steps)
+        print(json.dumps({"ragged": rag}), file=sys.stderr, flush=True)
     copy_gbps = copy_bandwidth(torch)
     cpu = None if a.no_cpu else cpu_baseline()
     cfg = bb.parse_fft_descriptor(a.desc)
@@ -131,9 +203,10 @@ def main():
                                f"x K={cfg.shape[2]}, out-of-place, uniform[-1,1) complex",
                    "descriptor": a.desc, "HBM_GBps": round(head["GBps"], 1),
                    "hbm_copy_GBps": round(copy_gbps, 1), "kernel": head["kernel"]["kernel"],
-                   "sweep": sweep},
+                   "sweep": sweep, "ragged_serving": rag},
         "roofline": {"bound": "hbm", "achieved": round(head["GBps"], 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(head["frac"], 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(head["frac"], 4),
+                     "traffic": MEASURED_TRAFFIC_BYTES.get(head["kernel"]["kernel"])},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -23,6 +23,9 @@ EXPORTED = [
     "bbfft_plan_create", "bbfft_plan_create_from_descriptor", "bbfft_plan_execute",
     "bbfft_plan_num_launches", "bbfft_plan_launch_info", "bbfft_plan_destroy",
     "bbfft_device_info_query", "bbfft_aot_generate",
+    "bbfft_ragged_kernel_configuration", "bbfft_ragged_variant_configuration",
+    "bbfft_ragged_job_variants", "bbfft_ragged_job_table", "bbfft_generate_ragged_source",
+    "bbfft_ragged_plan_create", "bbfft_aot_generate_ragged",
 ]
 
 SUCCESS = 0
@@ -71,6 +74,23 @@ class CTuning(ctypes.Structure):
         ("has_flags", ctypes.c_int),
     ]
 
+
+class CRaggedJob(ctypes.Structure):
+    _fields_ = [
+        ("M", ctypes.c_size_t), ("N", ctypes.c_size_t), ("K", ctypes.c_size_t),
+        ("direction", ctypes.c_int),
+        ("in_offset", ctypes.c_size_t), ("out_offset", ctypes.c_size_t),
+    ]
+
+
+class CRaggedInfo(ctypes.Structure):
+    _fields_ = [
+        ("group_size", ctypes.c_size_t), ("lds_bytes", ctypes.c_size_t),
+        ("num_variants", ctypes.c_size_t), ("total_tiles", ctypes.c_ulonglong),
+    ]
+
+
+RAGGED_GRAPH = 1
 
 _lib = None
 
@@ -138,6 +158,20 @@ def load():
         "bbfft_device_info_query": (c_int, [c_int, c_char_p, c_size]),
         "bbfft_aot_generate": (c_int, [P(c_char_p), c_size, c_char_p, c_char_p, c_char_p, c_uint,
                                        c_uint, P(c_size)]),
+        "bbfft_ragged_kernel_configuration": (c_int, [c_int, P(CRaggedJob), c_size, c_char_p,
+                                                      P(CRaggedInfo), c_char_p, c_size]),
+        "bbfft_ragged_variant_configuration": (c_int, [c_int, P(CRaggedJob), c_size, c_char_p,
+                                                       c_size, P(CKernelConfig)]),
+        "bbfft_ragged_job_variants": (c_int, [c_int, P(CRaggedJob), c_size, c_char_p,
+                                              P(c_uint)]),
+        "bbfft_ragged_job_table": (c_int, [c_int, P(CRaggedJob), c_size, c_char_p,
+                                           P(ctypes.c_ulonglong), c_size, P(c_size)]),
+        "bbfft_generate_ragged_source": (c_int, [c_int, P(CRaggedJob), c_size, c_char_p,
+                                                 c_char_p, c_size, P(c_size)]),
+        "bbfft_ragged_plan_create": (c_int, [c_int, P(CRaggedJob), c_size, c_uint, c_void_p,
+                                             c_void_p, P(c_void_p)]),
+        "bbfft_aot_generate_ragged": (c_int, [c_int, P(CRaggedJob), c_size, c_char_p, c_char_p,
+                                              c_char_p]),
     }
     for name, (res, args) in protos.items():
         fn = getattr(L, name)

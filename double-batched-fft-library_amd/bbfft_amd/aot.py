@@ -14,7 +14,11 @@ import sys
 _PKG_PARENT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def generate(descriptors, out_dir, device_info=None, arch="gfx950", jobs=8):
+def generate(descriptors, out_dir, device_info=None, arch="gfx950", jobs=8,
+             ragged_serving=False):
+    """Archives of every kernel the descriptors need; with ragged_serving also the persistent
+    kernel of the seeded mixed serving workload (bbfft_amd.ragged.synthetic_serving_jobs) and
+    its per-variant plain kernels, in OUT_DIR/kernels_ragged_serving.bin."""
     from . import GFX950_INFO
     device_info = device_info or GFX950_INFO
     os.makedirs(out_dir, exist_ok=True)
@@ -32,6 +36,11 @@ def generate(descriptors, out_dir, device_info=None, arch="gfx950", jobs=8):
         cmd = [sys.executable, "-m", "bbfft_amd.aot", "--shard", str(shard), "--num-shards",
                str(jobs), "-d", device_info, "-a", arch, "--out-file", out, "@" + listing]
         procs.append(subprocess.Popen(cmd, env=env))
+    if ragged_serving:
+        out = os.path.join(out_dir, "kernels_ragged_serving.bin")
+        cmd = [sys.executable, "-m", "bbfft_amd.aot", "--ragged-serving", "-d", device_info,
+               "-a", arch, "--out-file", out]
+        procs.append(subprocess.Popen(cmd, env=env))
     rc = [p.wait() for p in procs]
     if any(rc):
         raise RuntimeError(f"AOT generation failed (exit codes {rc})")
@@ -48,8 +57,16 @@ def main(argv=None):
     ap.add_argument("-j", "--jobs", type=int, default=8)
     ap.add_argument("--shard", type=int, default=None)
     ap.add_argument("--num-shards", type=int, default=1)
-    ap.add_argument("descriptors", nargs="+")
+    ap.add_argument("--ragged-serving", action="store_true",
+                    help="compile the ragged kernel of the seeded serving workload to --out-file")
+    ap.add_argument("descriptors", nargs="*")
     a = ap.parse_args(argv)
+    if a.ragged_serving:
+        from . import GFX950_INFO, ragged
+        ragged.aot_generate(ragged.synthetic_serving_jobs(), a.out_file,
+                            device_info=a.device_info or GFX950_INFO, arch=a.arch)
+        print(f"ragged serving kernel -> {a.out_file}", flush=True)
+        return 0
     descs = []
     for d in a.descriptors:
         if d.startswith("@"):

@@ -169,6 +169,22 @@ fft1d_configuration kernel_config_of(bbfft_configuration const *cfg, unsigned pa
                            tuning_of(tuning));
 }
 
+ragged_configuration ragged_of(int prec, bbfft_ragged_job const *jobs, size_t num_jobs) {
+    require(jobs != nullptr || num_jobs == 0, "null jobs");
+    require(prec == 4 || prec == 8, "precision must be 4 or 8");
+    ragged_configuration r;
+    r.fp = static_cast<precision>(prec);
+    r.jobs.resize(num_jobs);
+    for (size_t j = 0; j < num_jobs; ++j) {
+        require(jobs[j].direction == -1 || jobs[j].direction == 1,
+                "ragged job direction must be -1 or +1");
+        r.jobs[j] = ragged_job{jobs[j].M, jobs[j].N, jobs[j].K,
+                               static_cast<direction>(jobs[j].direction), jobs[j].in_offset,
+                               jobs[j].out_offset};
+    }
+    return r;
+}
+
 } // namespace
 
 extern "C" {
@@ -470,6 +486,126 @@ void bbfft_plan_destroy(bbfft_plan_t plan) { delete plan; }
 
 int bbfft_device_info_query(int device, char *buf, size_t len) {
     return guarded([&] { copy_out(hip::get_device_info(device).to_string(), buf, len); });
+}
+
+int bbfft_ragged_kernel_configuration(int prec, const bbfft_ragged_job *jobs, size_t num_jobs,
+                                      const char *device_info, bbfft_ragged_info *info,
+                                      char *name, size_t name_len) {
+    return guarded([&] {
+        require(device_info && info, "null argument");
+        auto rk = configure_ragged(ragged_of(prec, jobs, num_jobs), parse_device_info(device_info));
+        info->group_size = rk.group_size;
+        info->lds_bytes = rk.lds_bytes();
+        info->num_variants = rk.variants.size();
+        info->total_tiles = rk.total_tiles();
+        if (name) {
+            copy_out(rk.identifier(), name, name_len);
+        }
+    });
+}
+
+int bbfft_ragged_variant_configuration(int prec, const bbfft_ragged_job *jobs, size_t num_jobs,
+                                       const char *device_info, size_t variant,
+                                       bbfft_kernel_config *out) {
+    return guarded([&] {
+        require(device_info && out, "null argument");
+        auto rk = configure_ragged(ragged_of(prec, jobs, num_jobs), parse_device_info(device_info));
+        require(variant < rk.variants.size(), "variant index out of range");
+        auto const &kc = rk.variants[variant];
+        configuration pc{1, {kc.M, kc.N, 0}, kc.fp, static_cast<direction>(kc.direction),
+                         transform_type::c2c};
+        fill_kernel_config(kc, pc, out, nullptr, 0);
+    });
+}
+
+int bbfft_ragged_job_variants(int prec, const bbfft_ragged_job *jobs, size_t num_jobs,
+                              const char *device_info, unsigned *job_variant) {
+    return guarded([&] {
+        require(device_info && job_variant, "null argument");
+        auto rk = configure_ragged(ragged_of(prec, jobs, num_jobs), parse_device_info(device_info));
+        std::copy(rk.job_variant.begin(), rk.job_variant.end(), job_variant);
+    });
+}
+
+int bbfft_ragged_job_table(int prec, const bbfft_ragged_job *jobs, size_t num_jobs,
+                           const char *device_info, unsigned long long *table, size_t len,
+                           size_t *needed) {
+    return guarded([&] {
+        require(device_info != nullptr, "null argument");
+        auto r = ragged_of(prec, jobs, num_jobs);
+        auto t = ragged_job_table(r, configure_ragged(r, parse_device_info(device_info)));
+        if (needed) {
+            *needed = t.size();
+        }
+        if (table) {
+            require(len >= t.size(), "table buffer too small");
+            std::copy(t.begin(), t.end(), table);
+        }
+    });
+}
+
+int bbfft_generate_ragged_source(int prec, const bbfft_ragged_job *jobs, size_t num_jobs,
+                                 const char *device_info, char *buf, size_t len,
+                                 size_t *needed) {
+    return guarded([&] {
+        require(device_info != nullptr, "null argument");
+        auto rk = configure_ragged(ragged_of(prec, jobs, num_jobs), parse_device_info(device_info));
+        std::ostringstream os;
+        generate_ragged(os, rk);
+        auto s = os.str();
+        if (needed) {
+            *needed = s.size() + 1;
+        }
+        if (buf) {
+            copy_out(s, buf, len);
+        }
+    });
+}
+
+int bbfft_ragged_plan_create(int prec, const bbfft_ragged_job *jobs, size_t num_jobs,
+                             unsigned flags, bbfft_stream_t stream, bbfft_cache_t cache,
+                             bbfft_plan_t *plan) {
+    return guarded([&] {
+        require(plan != nullptr, "null argument");
+        require((flags & ~unsigned(BBFFT_RAGGED_GRAPH)) == 0, "unknown ragged plan flags");
+        auto r = ragged_of(prec, jobs, num_jobs);
+        int device = 0;
+        BBFFT_HIP_CHECK(hipGetDevice(&device));
+        auto impl = std::make_shared<hip_detail::ragged_plan>(
+            r, static_cast<hipStream_t>(stream), device, cache,
+            (flags & BBFFT_RAGGED_GRAPH) != 0);
+        *plan = new bbfft_plan_s{std::move(impl)};
+    });
+}
+
+int bbfft_aot_generate_ragged(int prec, const bbfft_ragged_job *jobs, size_t num_jobs,
+                              const char *device_info, const char *arch, const char *out_path) {
+    return guarded([&] {
+        require(device_info && arch && out_path, "null argument");
+        auto rk = configure_ragged(ragged_of(prec, jobs, num_jobs), parse_device_info(device_info));
+        std::ostringstream os;
+        generate_ragged(os, rk);
+        std::vector<hip::archive_entry> entries(2);
+        entries[0].arch = arch;
+        entries[0].kernel_names.push_back(rk.identifier());
+        entries[0].code = hip::compile_to_native(os.str(), arch);
+        // The plain kernel of each variant with a K tile of 1 and no access hints (the same
+        // arithmetic per job).
+        std::ostringstream ps;
+        std::set<std::string> seen;
+        for (auto const &v : rk.variants) {
+            auto p = v;
+            p.Kb = 1;
+            p.flags = 0;
+            if (seen.insert(p.identifier()).second) {
+                generate_fft1d(ps, p);
+                entries[1].kernel_names.push_back(p.identifier());
+            }
+        }
+        entries[1].arch = arch;
+        entries[1].code = hip::compile_to_native(ps.str(), arch);
+        hip::write_archive(out_path, entries);
+    });
 }
 
 int bbfft_aot_generate(const char *const *descriptors, size_t num_descriptors,

@@ -5,6 +5,7 @@
 #include "../base/math.hpp"
 #include "bbfft/bad_configuration.hpp"
 #include "bbfft/detail/generator_impl.hpp"
+#include "heuristics.hpp"
 #include "tuning_table.hpp"
 
 #include <algorithm>
@@ -63,7 +64,7 @@ std::string fft1d_configuration::identifier() const {
     return oss.str();
 }
 
-namespace {
+namespace gen {
 
 /// Stage radices for a complex FFT of length n: one stage up to `single_max`, otherwise the
 /// fewest stages whose largest radix does not exceed `radix_max` (factors closest to n^(1/L)).
@@ -92,7 +93,7 @@ std::vector<int> default_factorization(std::size_t n, std::size_t single_max,
     return {static_cast<int>(n)};
 }
 
-} // namespace
+} // namespace gen
 
 fft1d_configuration configure_fft1d(configuration const &cfg, device_info const &info,
                                     fft1d_tuning const &tuning_in, unsigned variant) {
@@ -148,7 +149,7 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     } else {
         std::size_t single_max = real_bytes == 4 ? 16 : 16;
         std::size_t radix_max = real_bytes == 4 ? 32 : 16;
-        c.factorization = default_factorization(nfft, single_max, radix_max);
+        c.factorization = gen::default_factorization(nfft, single_max, radix_max);
     }
     for (auto f : c.factorization) {
         if (f > 64 && f != 1) {

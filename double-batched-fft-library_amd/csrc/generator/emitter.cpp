@@ -252,8 +252,8 @@ std::string ul(std::size_t v) { return std::to_string(v) + "ul"; }
 /// Emits one kernel.
 class kernel_gen {
   public:
-    kernel_gen(fft1d_configuration const &c, std::string name)
-        : c_(c), f_{c.fp}, name_(std::move(name)) {
+    kernel_gen(fft1d_configuration const &c, std::string name, std::size_t group_size = 0)
+        : c_(c), f_{c.fp}, name_(std::move(name)), group_size_(group_size) {
         nfft_ = c.N_fft();
         L_ = c.factorization.size();
         real_ = c.type != transform_type::c2c;
@@ -273,6 +273,15 @@ class kernel_gen {
     std::string generate() {
         emit_tables();
         emit_kernel();
+        return w_.str();
+    }
+
+    /// Device function computing one tile: `NAME(in, out, K, tile, lds_buf)`, for the
+    /// persistent ragged kernel. Work-items beyond Mb * Nt * Kb of a `group_size` work-group
+    /// take part in the barriers only.
+    std::string generate_body() {
+        emit_tables();
+        emit_body();
         return w_.str();
     }
 
@@ -392,6 +401,43 @@ class kernel_gen {
             w_.line(T2() + "* col = lds_buf + ml + " + u(c_.Mb * np_) + " * kl;");
         }
         w_.open("for (unsigned long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)");
+        emit_tile();
+        if (use_lds_) {
+            w_.line("__syncthreads();");
+        }
+        w_.close();
+        w_.close();
+        w_.line("}");
+    }
+
+    void emit_body() {
+        w_.line("static __device__ __forceinline__ void " + name_ + "(const " + in_type() +
+                "* in, " + out_type() + "* out, unsigned long K, unsigned long tile, " + T2() +
+                "* lds_buf, unsigned lid) {");
+        w_.open("");
+        w_.line("const unsigned ml = lid % " + u(c_.Mb) + ";");
+        w_.line("const unsigned t = (lid / " + u(c_.Mb) + ") % " + u(c_.Nt) + ";");
+        w_.line("const unsigned kl = lid / " + u(c_.Mb * c_.Nt) + ";");
+        if (group_size_ > c_.work_group_size()) {
+            band_ = "inb";
+            w_.line("const bool inb = kl < " + u(c_.Kb) + ";");
+        }
+        if (pairs_) {
+            w_.line("const unsigned long kcols = (K + 1ul) / 2ul;");
+        } else {
+            w_.line("const unsigned long kcols = K;");
+        }
+        if (use_lds_) {
+            std::string klc = band_.empty() ? "kl" : "(inb ? kl : 0u)";
+            w_.line(T2() + "* col = lds_buf + ml + " + u(c_.Mb * np_) + " * " + klc + ";");
+        }
+        emit_tile();
+        w_.close();
+        w_.line("}");
+    }
+
+    /// One (M tile, K tile) of work: `tile` selects it.
+    void emit_tile() {
         if (MG_ == 1) {
             w_.line("const unsigned long m = ml;");
             w_.line("const unsigned long kc = tile * " + ul(c_.Kb) + " + kl;");
@@ -403,6 +449,9 @@ class kernel_gen {
         std::string act = "kc < kcols";
         if (MG_ * c_.Mb != c_.M) {
             act = "m < " + ul(c_.M) + " && " + act;
+        }
+        if (!band_.empty()) {
+            act = band_ + " && " + act;
         }
         w_.line("const bool active = " + act + ";");
         if (pairs_) {
@@ -424,12 +473,6 @@ class kernel_gen {
         if (c_.type == transform_type::r2c) {
             emit_r2c_post();
         }
-        if (use_lds_) {
-            w_.line("__syncthreads();");
-        }
-        w_.close();
-        w_.close();
-        w_.line("}");
     }
 
     /// Offset (elements) of FFT index n in the input column along mode 1.
@@ -521,6 +564,8 @@ class kernel_gen {
             std::string cond = guard[b];
             if (src_global) {
                 cond = cond.empty() ? "active" : "active && " + cond;
+            } else if (!band_.empty()) {
+                cond = cond.empty() ? band_ : band_ + " && " + cond;
             }
             if (!cond.empty()) {
                 w_.open("if (" + cond + ")");
@@ -554,8 +599,12 @@ class kernel_gen {
             }
             // The butterfly is computed unconditionally when only the store is predicated;
             // scoping keeps the SSA temporaries local.
-            if (!guard[b].empty()) {
-                w_.open("if (" + guard[b] + ")");
+            std::string ccond = guard[b];
+            if (!band_.empty()) {
+                ccond = ccond.empty() ? band_ : band_ + " && " + ccond;
+            }
+            if (!ccond.empty()) {
+                w_.open("if (" + ccond + ")");
             } else {
                 w_.open("");
             }
@@ -694,6 +743,8 @@ class kernel_gen {
     fft1d_configuration const &c_;
     fp_helper f_;
     std::string name_;
+    std::size_t group_size_ = 0;
+    std::string band_;
     writer w_;
     std::size_t nfft_ = 0, L_ = 0, MG_ = 1, np_ = 0;
     unsigned pad_shift_ = 0;
@@ -760,6 +811,18 @@ std::string callback_source(user_module const &um) {
     return src + "\n";
 }
 } // namespace
+
+void generate_fft1d_body(std::ostream &os, fft1d_configuration const &cfg,
+                         std::string const &name, std::size_t group_size) {
+    if (cfg.load_function || cfg.store_function) {
+        throw bad_configuration("callbacks are not supported in ragged batches");
+    }
+    if (cfg.type != transform_type::c2c) {
+        throw bad_configuration("ragged batches support c2c transforms only");
+    }
+    kernel_gen g(cfg, name, group_size);
+    os << g.generate_body();
+}
 
 void generate_fft1d(std::ostream &os, fft1d_configuration const &cfg, std::string_view name,
                     user_module const *callbacks) {

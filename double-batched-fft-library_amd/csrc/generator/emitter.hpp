@@ -1,0 +1,22 @@
+// Internal entry points of the HIP source generator.
+#ifndef BBFFT_AMD_EMITTER_HPP
+#define BBFFT_AMD_EMITTER_HPP
+
+#include "bbfft/detail/generator_impl.hpp"
+
+#include <cstddef>
+#include <iosfwd>
+#include <string>
+
+namespace bbfft {
+
+/// Complex helper functions shared by all kernels of a source (include-guarded).
+std::string hip_prelude();
+/// One tile of cfg's kernel as `static __device__ void name(in, out, K, tile, lds)` for a
+/// work-group of `group_size` work-items (>= cfg.work_group_size()).
+void generate_fft1d_body(std::ostream &os, fft1d_configuration const &cfg,
+                         std::string const &name, std::size_t group_size);
+
+} // namespace bbfft
+
+#endif // BBFFT_AMD_EMITTER_HPP

@@ -7,6 +7,7 @@
 #include "bbfft/detail/generator_impl.hpp"
 #include "bbfft/detail/plan_impl.hpp"
 #include "bbfft/jit_cache.hpp"
+#include "bbfft/ragged.hpp"
 #include "bbfft/shared_handle.hpp"
 
 #include <hip/hip_runtime_api.h>
@@ -68,6 +69,40 @@ class nd_plan final : public plan_base {
     hipStream_t stream_;
     std::vector<std::unique_ptr<fft1d_plan>> passes_;
     void *tmp_ = nullptr;
+};
+
+/// Ragged batch: one persistent kernel over a device job table, optionally replayed from a
+/// HIP graph that holds the launch.
+class ragged_plan final : public plan_base {
+  public:
+    ragged_plan(ragged_configuration const &cfg, hipStream_t stream, int device,
+                jit_cache *cache, bool use_graph);
+    ~ragged_plan();
+    ragged_plan(ragged_plan const &) = delete;
+    ragged_plan &operator=(ragged_plan const &) = delete;
+    void execute(void const *in, void *out, hipEvent_t signal, std::uint32_t nwait,
+                 hipEvent_t *wait) override;
+    std::vector<launch_info> launches() const override;
+    ragged_kernel_configuration const &kernel_config() const { return rk_; }
+
+  private:
+    void launch(void const *in, void *out, hipStream_t stream);
+    void release_graph();
+
+    hipStream_t stream_;
+    int device_;
+    ragged_kernel_configuration rk_;
+    std::string name_;
+    shared_handle<module_handle_t> mod_;
+    hipFunction_t fn_ = nullptr;
+    void *table_ = nullptr;
+    unsigned long njobs_ = 0;
+    unsigned grid_ = 0, block_ = 0;
+    std::size_t in_end_ = 0, out_end_ = 0; ///< complex elements touched from in / out
+    bool use_graph_ = false;
+    hipGraphExec_t exec_ = nullptr;
+    void const *graph_in_ = nullptr;
+    void *graph_out_ = nullptr;
 };
 
 std::shared_ptr<plan_base> create(configuration const &cfg, hipStream_t stream,

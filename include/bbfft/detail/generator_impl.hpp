@@ -9,9 +9,11 @@
 #include "bbfft/configuration.hpp"
 #include "bbfft/device_info.hpp"
 #include "bbfft/export.hpp"
+#include "bbfft/ragged.hpp"
 
 #include <array>
 #include <cstddef>
+#include <cstdint>
 #include <iosfwd>
 #include <string>
 #include <string_view>
@@ -99,6 +101,45 @@ BBFFT_EXPORT fft1d_configuration configure_fft1d(configuration const &cfg,
 BBFFT_EXPORT void generate_fft1d(std::ostream &os, fft1d_configuration const &cfg,
                                  std::string_view name = {},
                                  user_module const *callbacks = nullptr);
+
+/**
+ * @brief Persistent kernel of a ragged batch.
+ *
+ * Every distinct (N, M, direction) of the jobs is a variant: an fft1d_configuration whose
+ * factorisation, work-items per transform and M tile come from configure_fft1d (tuning table or
+ * heuristics) and whose K tile is widened so that Mb * Nt * Kb fills the common work-group of
+ * `group_size` work-items. The tiles of all jobs form one linear tile space; work-group g
+ * processes tiles g, g + grid, ... and finds each tile's job by binary search in the job table.
+ */
+struct BBFFT_EXPORT ragged_kernel_configuration {
+    precision fp;
+    std::size_t group_size;                    ///< work-items per work-group
+    std::vector<fft1d_configuration> variants; ///< one per distinct (N, M, direction)
+    std::vector<unsigned> job_variant;         ///< variant of each job
+    std::vector<std::size_t> order;            ///< job index at each table position
+    std::vector<std::uint64_t> tile_begin;     ///< first tile of each table position, + total
+
+    std::uint64_t total_tiles() const { return tile_begin.empty() ? 0 : tile_begin.back(); }
+    /// Kernel symbol name; depends on the variants and the work-group size only, so the compiled
+    /// module serves any job list with the same variants.
+    std::string identifier() const;
+    /// Bytes of LDS (the largest of the variants).
+    std::size_t lds_bytes() const;
+};
+
+/// Configure the persistent kernel for a ragged batch (validates the jobs).
+BBFFT_EXPORT ragged_kernel_configuration configure_ragged(ragged_configuration const &cfg,
+                                                          device_info const &info);
+/// Emit the HIP source of the persistent kernel `name(in, out, table, njobs)`.
+BBFFT_EXPORT void generate_ragged(std::ostream &os, ragged_kernel_configuration const &rk,
+                                  std::string_view name = {});
+/**
+ * @brief Device job table: tile_begin[0..njobs] (njobs + 1 entries, the last is the total),
+ * followed by njobs records {in_offset, out_offset, K, variant} in table order (offsets in
+ * complex elements).
+ */
+BBFFT_EXPORT std::vector<std::uint64_t> ragged_job_table(ragged_configuration const &cfg,
+                                                         ragged_kernel_configuration const &rk);
 
 } // namespace bbfft
 

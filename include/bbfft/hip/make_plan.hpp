@@ -8,6 +8,7 @@
 #include "bbfft/export.hpp"
 #include "bbfft/jit_cache.hpp"
 #include "bbfft/plan.hpp"
+#include "bbfft/ragged.hpp"
 
 #include <hip/hip_runtime_api.h>
 
@@ -24,6 +25,16 @@ using hip_plan = plan_unmanaged_event<hipEvent_t>;
  */
 BBFFT_EXPORT auto make_plan(configuration const &cfg, hipStream_t stream,
                             jit_cache *cache = nullptr) -> hip_plan;
+
+/**
+ * @brief Plan for a ragged batch (extension): all jobs in one persistent kernel launch.
+ *
+ * execute(in, out) runs every job on its slice of in / out. With use_graph the launch is
+ * captured into a HIP graph (once per distinct (in, out) pair) and replayed.
+ */
+BBFFT_EXPORT auto make_ragged_plan(ragged_configuration const &cfg, hipStream_t stream,
+                                   jit_cache *cache = nullptr, bool use_graph = true)
+    -> hip_plan;
 
 } // namespace bbfft
 

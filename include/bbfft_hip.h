@@ -179,6 +179,60 @@ BBFFT_EXPORT_C void bbfft_plan_destroy(bbfft_plan_t plan);
 /* Device info of a HIP device as a device-info string (src/cl/device.cpp:12-40 pattern). */
 BBFFT_EXPORT_C int bbfft_device_info_query(int device, char *buf, size_t len);
 
+/*
+ * Ragged batches (extension; no reference counterpart: the reference runs one plan per shape,
+ * src/common/algorithm/factor2_slm_fft.hpp:92-203). A job is a c2c M x N x K tensor with packed
+ * column-major strides {1, M, M*N}; in_offset / out_offset count complex elements from the
+ * pointers given to bbfft_plan_execute. precision: 4 or 8; direction: -1 forward, +1 backward.
+ */
+typedef struct bbfft_ragged_job {
+    size_t M, N, K;
+    int direction;
+    size_t in_offset, out_offset;
+} bbfft_ragged_job;
+
+/* Persistent kernel of a ragged batch (include/bbfft/detail/generator_impl.hpp:
+ * ragged_kernel_configuration). */
+typedef struct bbfft_ragged_info {
+    size_t group_size, lds_bytes, num_variants;
+    unsigned long long total_tiles;
+} bbfft_ragged_info;
+
+enum { BBFFT_RAGGED_GRAPH = 1 }; /* replay the launch from a captured HIP graph */
+
+BBFFT_EXPORT_C int bbfft_ragged_kernel_configuration(int precision, const bbfft_ragged_job *jobs,
+                                                     size_t num_jobs, const char *device_info,
+                                                     bbfft_ragged_info *info, char *name,
+                                                     size_t name_len);
+/* Tiling of variant `variant` (0 <= variant < num_variants); K is left 0. */
+BBFFT_EXPORT_C int bbfft_ragged_variant_configuration(int precision,
+                                                      const bbfft_ragged_job *jobs,
+                                                      size_t num_jobs, const char *device_info,
+                                                      size_t variant, bbfft_kernel_config *out);
+/* Variant of each job (job_variant[num_jobs]). */
+BBFFT_EXPORT_C int bbfft_ragged_job_variants(int precision, const bbfft_ragged_job *jobs,
+                                             size_t num_jobs, const char *device_info,
+                                             unsigned *job_variant);
+/* Device job table (num_jobs + 1 + 4 * num_jobs 64-bit words), as the kernel reads it. */
+BBFFT_EXPORT_C int bbfft_ragged_job_table(int precision, const bbfft_ragged_job *jobs,
+                                          size_t num_jobs, const char *device_info,
+                                          unsigned long long *table, size_t len, size_t *needed);
+BBFFT_EXPORT_C int bbfft_generate_ragged_source(int precision, const bbfft_ragged_job *jobs,
+                                                size_t num_jobs, const char *device_info,
+                                                char *buf, size_t len, size_t *needed);
+/* flags: BBFFT_RAGGED_GRAPH or 0. The plan is executed and destroyed like any other plan. */
+BBFFT_EXPORT_C int bbfft_ragged_plan_create(int precision, const bbfft_ragged_job *jobs,
+                                            size_t num_jobs, unsigned flags,
+                                            bbfft_stream_t stream, bbfft_cache_t cache,
+                                            bbfft_plan_t *plan);
+/* Ahead-of-time compilation of the ragged kernel for these jobs' variants, plus the plain
+ * kernel of every variant with a K tile of 1 (what bbfft_plan_create_tuned builds for the
+ * variant's factorisation, Nt, Mb and lds_pad with Kb = 1 and flags 0: per-job launches with
+ * the same arithmetic). The archive loads with bbfft_cache_load_aot. */
+BBFFT_EXPORT_C int bbfft_aot_generate_ragged(int precision, const bbfft_ragged_job *jobs,
+                                             size_t num_jobs, const char *device_info,
+                                             const char *arch, const char *out_path);
+
 /* bbfft-aot-generate (tools/aot/main.cpp:22-60): generate every kernel the descriptors need for
  * device_info, compile them with hiprtc for `arch` (e.g. "gfx950") and write an archive to
  * out_path. Kernel i of the de-duplicated list is compiled only when i % num_shards == shard, so

@@ -14,14 +14,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _BUILD = os.path.join(_HERE, "_build")
 
 
-def compile_kernel(source, kernel_name, in_type, out_type):
-    """Compile `source` with the emulation shim; returns a ctypes launcher."""
+def _build_so(tu):
     os.makedirs(_BUILD, exist_ok=True)
-    tu = ('#include "hip_emu.hpp"\n' + source + "\n"
-          'extern "C" int emu_launch(const void* in, void* out, unsigned long K, unsigned grid, '
-          "unsigned block) {\n"
-          f"  return emu::launch(grid, block, [&] {{ {kernel_name}(({in_type}*)in, "
-          f"({out_type}*)out, K); }});\n}}\n")
     h = hashlib.sha1(tu.encode()).hexdigest()[:16]
     so = os.path.join(_BUILD, f"k_{h}.so")
     if not os.path.exists(so):
@@ -32,7 +26,17 @@ def compile_kernel(source, kernel_name, in_type, out_type):
         subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-w", "-I", _HERE, src,
                         "-o", tmp], check=True)
         os.replace(tmp, so)
-    lib = ctypes.CDLL(so)
+    return ctypes.CDLL(so)
+
+
+def compile_kernel(source, kernel_name, in_type, out_type):
+    """Compile `source` with the emulation shim; returns a ctypes launcher."""
+    tu = ('#include "hip_emu.hpp"\n' + source + "\n"
+          'extern "C" int emu_launch(const void* in, void* out, unsigned long K, unsigned grid, '
+          "unsigned block) {\n"
+          f"  return emu::launch(grid, block, [&] {{ {kernel_name}(({in_type}*)in, "
+          f"({out_type}*)out, K); }});\n}}\n")
+    lib = _build_so(tu)
     fn = lib.emu_launch
     fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulong, ctypes.c_uint, ctypes.c_uint]
     fn.restype = ctypes.c_int
@@ -69,6 +73,32 @@ def run(bbfft, cfg, x, out_size, grid_cap=None, device_info=None, pass_index=0,
     xin = np.ascontiguousarray(x, dtype=npr if t == 1 else npc)
     out = np.zeros(out_size, dtype=npr if t == 2 else npc)
     rc = fn(xin.ctypes.data, out.ctypes.data, K, grid, kc["work_group_size"])
+    if rc != 0:
+        raise RuntimeError("divergent barrier in emulated kernel")
+    return out
+
+
+def run_ragged(ragged, jobs, x, out_size, precision=4, grid=7, in_place=False):
+    """Execute the persistent ragged kernel for `jobs` on host arrays (`grid` work-groups)."""
+    kc = ragged.kernel_configuration(jobs, precision)
+    src = ragged.generate_source(jobs, precision)
+    table = ragged.job_table(jobs, precision)
+    c2 = "float2" if precision == 4 else "double2"
+    npc = np.complex64 if precision == 4 else np.complex128
+    name = kc["name"]
+    tu = ('#include "hip_emu.hpp"\n' + src + "\n"
+          'extern "C" int emu_launch_ragged(const void* in, void* out, const unsigned long* t, '
+          "unsigned long njobs, unsigned grid, unsigned block) {\n"
+          f"  return emu::launch(grid, block, [&] {{ {name}((const {c2}*)in, ({c2}*)out, t, "
+          "njobs); });\n}\n")
+    fn = _build_so(tu).emu_launch_ragged
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulong,
+                   ctypes.c_uint, ctypes.c_uint]
+    fn.restype = ctypes.c_int
+    xin = np.ascontiguousarray(x, dtype=npc)
+    out = xin if in_place else np.zeros(out_size, dtype=npc)
+    rc = fn(xin.ctypes.data, out.ctypes.data, table.ctypes.data, len(jobs), grid,
+            kc["group_size"])
     if rc != 0:
         raise RuntimeError("divergent barrier in emulated kernel")
     return out
