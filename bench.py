@@ -5,7 +5,9 @@ Workload (headline): plain c2c forward, fp32, M=8 x N=256 x K=32768 (descriptor
 plan.execute over the whole tensor. Also reported: the sweep N in {16..512, 60, 105, 243, 384}
 for fp32 and fp64 (same M, K), a device-to-device copy bandwidth, and the CPU baseline (the
 oracle's C port of the reference algorithm on a bounded sample, host cores), and the ragged
-serving workload (4096 mixed jobs in one persistent launch vs per-job plain launches).
+serving workload (4096 mixed jobs in one persistent launch vs per-job plain launches) and the
+N=512 precision variant (fp16 / bf16 storage with the MFMA-split radix-16 stage, errors against
+an FP64 FFT of the same values).
 
 Accounting (reference benchmark/adapter.hpp:51-52): flops = 5 N log2 N M K, bytes = 2 M N K
 sizeof(complex). Timing: HIP events on the plan's stream around the timed steps.
@@ -134,6 +136,68 @@ def ragged_serving(bb, torch, cache, steps):
             "lds_bytes": launch["lds_bytes"]}
 
 
+def mfma_split(bb, torch, cache, steps):
+    """N=512 c2c forward, M=4 x K=262144: fp16 / bf16 storage through the MFMA-split kernel
+    (radix-16 stage as a DFT-matrix GEMM on the matrix cores) and through the vector kernel with
+    the same storage, beside the plain fp32 path; each with its error against an FP64 FFT
+    (torch.fft on complex128) of the same input values."""
+    M, N, K = 4, 512, 262144
+    n = M * N * K
+    g = torch.Generator(device="cuda")
+    g.manual_seed(512)
+    re = torch.rand(n, generator=g, device="cuda") * 2 - 1
+    im = torch.rand(n, generator=g, device="cuda") * 2 - 1
+    cfg = bb.parse_fft_descriptor(f"scfo{M}.{N}*{K}")
+    flops = 5.0 * N * math.log2(N) * M * K
+    out = {}
+
+    def run(plan, x, y, nbytes, ref):
+        plan.execute(x, y)
+        torch.cuda.synchronize()
+        d = y.to(torch.complex128) - ref
+        rel = (torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref)).item()
+        mx = d.abs().max().item()
+        del d
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(steps):
+            plan.execute(x, y)
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e) / steps
+        return {"ms": round(ms, 4), "GFLOPs": round(flops / ms / 1e6, 1),
+                "GBps": round(nbytes / ms / 1e6, 1),
+                "frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBPS, 4),
+                "rel_l2_vs_fp64": float(f"{rel:.3e}"), "max_abs_vs_fp64": float(f"{mx:.3e}")}
+
+    for storage, hdt in (("f16", torch.float16), ("bf16", torch.bfloat16)):
+        pair = torch.stack([re.to(hdt), im.to(hdt)], dim=1).contiguous()
+        x64 = torch.complex(pair[:, 0].double(), pair[:, 1].double())
+        ref = torch.fft.fft(x64.view(K, N, M), dim=1).reshape(-1)
+        packed = pair.view(torch.int32).reshape(n)
+        y = torch.empty(n, dtype=torch.complex64, device="cuda")
+        nbytes = n * (4.0 + 8.0)
+        for mf in (True, False):
+            plan = bb.make_half_input_plan(cfg, storage, mf, cache=cache)
+            key = f"{storage}_{'mfma_split' if mf else 'vector'}"
+            out[key] = run(plan, packed, y, nbytes, ref)
+            out[key]["kernel"] = plan.launches()[0]["kernel"]
+        if storage == "f16":
+            # plain fp32 storage of the same (f16-exact) values
+            x32 = x64.to(torch.complex64)
+            plan = bb.make_plan(cfg, cache=cache)
+            out["plain_f32"] = run(plan, x32, y, n * 16.0, ref)
+            out["plain_f32"]["kernel"] = plan.launches()[0]["kernel"]
+            del x32
+        del pair, x64, ref, packed, y
+        torch.cuda.empty_cache()
+    out["speedup_f16_mfma_vs_plain_f32"] = round(out["plain_f32"]["ms"] /
+                                                out["f16_mfma_split"]["ms"], 3)
+    out["speedup_bf16_mfma_vs_plain_f32"] = round(out["plain_f32"]["ms"] /
+                                                 out["bf16_mfma_split"]["ms"], 3)
+    return out
+
+
 def cpu_baseline(budget_s=15.0):
     """Oracle port of the reference algorithm on host cores, bounded sample of the workload."""
     try:
@@ -151,6 +215,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ragged", action="store_true")
+    ap.add_argument("--no-mfma", action="store_true")
     ap.add_argument("--desc", default="scfo8.256*32768")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (counter calibration)")
@@ -179,6 +244,10 @@ def main():
                               "GFLOPs": round(r["gflops"], 1), "GBps": round(r["GBps"], 1),
                               "frac": round(r["frac"], 4)})
                 print(json.dumps(sweep[-1]), file=sys.stderr, flush=True)
+    mf = None
+    if not a.no_mfma:
+        mf = mfma_split(bb, torch, cache, a.steps)
+        print(json.dumps({"mfma_split": mf}), file=sys.stderr, flush=True)
     rag = None
     if not a.no_ragged:
         rag = ragged_serving(bb, torch, cache, a.steps)
@@ -203,7 +272,7 @@ def main():
                                f"x K={cfg.shape[2]}, out-of-place, uniform[-1,1) complex",
                    "descriptor": a.desc, "HBM_GBps": round(head["GBps"], 1),
                    "hbm_copy_GBps": round(copy_gbps, 1), "kernel": head["kernel"]["kernel"],
-                   "sweep": sweep, "ragged_serving": rag},
+                   "sweep": sweep, "mfma_split_n512": mf, "ragged_serving": rag},
         "roofline": {"bound": "hbm", "achieved": round(head["GBps"], 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(head["frac"], 4),
                      "traffic": MEASURED_TRAFFIC_BYTES.get(head["kernel"]["kernel"])},

@@ -409,6 +409,34 @@ def make_adjoint_plan(cfg, stream=None, cache: Optional["Cache"] = None) -> "Pla
     return Plan(h, adjoint_configuration(cfg), stream)
 
 
+INPUT_F16 = _capi.INPUT_F16
+INPUT_BF16 = _capi.INPUT_BF16
+MFMA = _capi.MFMA
+
+
+def half_input_flags(storage="f16", mfma=True) -> int:
+    """Kernel-variant flags of a c2c fp32 plan reading "f16" or "bf16" (re, im) pairs."""
+    if storage not in ("f16", "bf16"):
+        raise ValueError("storage must be 'f16' or 'bf16'")
+    return (INPUT_F16 if storage == "f16" else INPUT_BF16) | (MFMA if mfma else 0)
+
+
+def make_half_input_plan(cfg, storage="f16", mfma=True, stream=None,
+                         cache: Optional["Cache"] = None) -> "Plan":
+    """1-D c2c fp32 plan whose input is stored as half-precision (re, im) pairs (4 bytes per
+    complex element, strides as in cfg) and whose output is fp32 complex; with mfma the radix-16
+    first stage runs on the matrix cores (split DFT matrix, fp32 accumulation)."""
+    if isinstance(cfg, str):
+        cfg = parse_fft_descriptor(cfg)
+    if stream is None:
+        stream = _current_stream()
+    h = ctypes.c_void_p()
+    _check(_capi.load().bbfft_plan_create_half_input(
+        ctypes.byref(cfg._to_c()), half_input_flags(storage, mfma), stream,
+        cache.handle if cache is not None else None, ctypes.byref(h)))
+    return Plan(h, cfg, stream)
+
+
 DEFAULT_AOT_DIR = os.path.join(os.path.dirname(_capi.DEFAULT_LIB), "aot")
 _default_cache = None
 

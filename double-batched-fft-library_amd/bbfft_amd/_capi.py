@@ -25,7 +25,7 @@ EXPORTED = [
     "bbfft_device_info_query", "bbfft_aot_generate",
     "bbfft_ragged_kernel_configuration", "bbfft_ragged_variant_configuration",
     "bbfft_ragged_job_variants", "bbfft_ragged_job_table", "bbfft_generate_ragged_source",
-    "bbfft_ragged_plan_create", "bbfft_aot_generate_ragged",
+    "bbfft_ragged_plan_create", "bbfft_aot_generate_ragged", "bbfft_plan_create_half_input",
 ]
 
 SUCCESS = 0
@@ -91,6 +91,9 @@ class CRaggedInfo(ctypes.Structure):
 
 
 RAGGED_GRAPH = 1
+INPUT_F16 = 8
+INPUT_BF16 = 16
+MFMA = 32
 
 _lib = None
 
@@ -172,6 +175,8 @@ def load():
                                              c_void_p, P(c_void_p)]),
         "bbfft_aot_generate_ragged": (c_int, [c_int, P(CRaggedJob), c_size, c_char_p, c_char_p,
                                               c_char_p]),
+        "bbfft_plan_create_half_input": (c_int, [P(CConfiguration), c_uint, c_void_p, c_void_p,
+                                                 P(c_void_p)]),
     }
     for name, (res, args) in protos.items():
         fn = getattr(L, name)

@@ -488,6 +488,20 @@ int bbfft_device_info_query(int device, char *buf, size_t len) {
     return guarded([&] { copy_out(hip::get_device_info(device).to_string(), buf, len); });
 }
 
+int bbfft_plan_create_half_input(const bbfft_configuration *cfg, unsigned flags,
+                                 bbfft_stream_t stream, bbfft_cache_t cache, bbfft_plan_t *plan) {
+    return guarded([&] {
+        require(cfg && plan, "null argument");
+        unsigned const allowed = BBFFT_INPUT_F16 | BBFFT_INPUT_BF16 | BBFFT_MFMA;
+        require((flags & ~allowed) == 0, "unknown half-input flags");
+        require((flags & (BBFFT_INPUT_F16 | BBFFT_INPUT_BF16)) != 0,
+                "one of BBFFT_INPUT_F16 / BBFFT_INPUT_BF16 is required");
+        auto impl = hip_detail::create_variant(from_c(*cfg), flags,
+                                               static_cast<hipStream_t>(stream), cache);
+        *plan = new bbfft_plan_s{std::move(impl)};
+    });
+}
+
 int bbfft_ragged_kernel_configuration(int prec, const bbfft_ragged_job *jobs, size_t num_jobs,
                                       const char *device_info, bbfft_ragged_info *info,
                                       char *name, size_t name_len) {

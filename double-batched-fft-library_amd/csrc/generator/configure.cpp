@@ -117,7 +117,12 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     bool const explicit_tuning = !tuning_in.factorization.empty() || tuning_in.Nt ||
                                  tuning_in.Mb || tuning_in.Kb || tuning_in.lds_pad ||
                                  tuning_in.no_pad || tuning_in.has_flags;
-    if (!explicit_tuning && info.type == device_type::gpu && info.max_subgroup_size() == 64) {
+    unsigned const all_flags = (tuning_in.has_flags ? tuning_in.flags : 0u) | variant;
+    bool const half_in = (all_flags & (fft1d_input_f16 | fft1d_input_bf16)) != 0;
+    bool const mfma = (all_flags & fft1d_mfma) != 0;
+    // The table holds kernels measured with fp32 storage.
+    if (!explicit_tuning && !half_in && info.type == device_type::gpu &&
+        info.max_subgroup_size() == 64) {
         gen::lookup_tuning(cfg.fp, cfg.type, N, M, tuning);
     }
 
@@ -134,6 +139,28 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     std::size_t const nfft = c.N_fft();
     std::size_t const real_bytes = static_cast<std::size_t>(cfg.fp);
 
+    if (half_in) {
+        if ((all_flags & fft1d_input_f16) && (all_flags & fft1d_input_bf16)) {
+            throw bad_configuration("Choose one half-precision input format");
+        }
+        if (cfg.type != transform_type::c2c || cfg.fp != precision::f32) {
+            throw bad_configuration("Half-precision input requires a c2c fp32 transform");
+        }
+        if (cfg.callbacks) {
+            throw bad_configuration("Half-precision input does not support callbacks");
+        }
+    }
+    if (mfma) {
+        if (!half_in) {
+            throw bad_configuration("The MFMA stage requires half-precision input");
+        }
+        if (nfft % 16 != 0 || nfft < 32) {
+            throw bad_configuration("The MFMA stage requires N divisible by 16 and N >= 32");
+        }
+        if (!tuning.factorization.empty() && tuning.factorization[0] != 16) {
+            throw bad_configuration("The MFMA stage requires a first radix of 16");
+        }
+    }
     if (!tuning.factorization.empty()) {
         std::size_t p = 1;
         for (auto f : tuning.factorization) {
@@ -150,6 +177,10 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
         std::size_t single_max = real_bytes == 4 ? 16 : 16;
         std::size_t radix_max = real_bytes == 4 ? 32 : 16;
         c.factorization = gen::default_factorization(nfft, single_max, radix_max);
+        if (mfma) {
+            c.factorization = gen::default_factorization(nfft / 16, 16, 16);
+            c.factorization.insert(c.factorization.begin(), 16);
+        }
     }
     for (auto f : c.factorization) {
         if (f > 64 && f != 1) {
@@ -158,7 +189,9 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
                                     std::to_string(N) + " is not supported");
         }
     }
-    int const rmax = *std::max_element(c.factorization.begin(), c.factorization.end());
+    // The MFMA stage does not use the work-items-per-transform split; later stages set it.
+    int const rmax = mfma ? *std::max_element(c.factorization.begin() + 1, c.factorization.end())
+                          : *std::max_element(c.factorization.begin(), c.factorization.end());
     c.Nt = tuning.Nt ? tuning.Nt : nfft / static_cast<std::size_t>(rmax);
 
     std::size_t const wg_max = std::min<std::size_t>(1024, info.max_work_group_size);
@@ -206,6 +239,15 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
         c.Kb = kb;
         while (c.Kb > 1 && c.lds_bytes() > lds_limit) {
             --c.Kb;
+        }
+    }
+    if (mfma) {
+        // Matrix instructions read all 64 lanes of a wave: whole waves only (the extra K
+        // columns are inactive when K is smaller).
+        std::size_t const step = 64 / std::gcd<std::size_t>(c.Mb * c.Nt, 64);
+        c.Kb = (c.Kb + step - 1) / step * step;
+        if (c.lds_bytes() > lds_limit) {
+            throw bad_configuration("MFMA kernel tile exceeds the LDS budget");
         }
     }
     if (c.work_group_size() > wg_max) {

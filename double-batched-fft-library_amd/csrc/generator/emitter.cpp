@@ -20,7 +20,11 @@
 #include "bbfft/detail/generator_impl.hpp"
 #include "roots.hpp"
 
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <ostream>
 #include <sstream>
 #include <string>
@@ -29,6 +33,64 @@
 namespace bbfft {
 
 namespace {
+
+/// Round to the nearest value with `mant` fraction bits and minimum normal exponent `emin`
+/// (ties to even); subnormals keep the quantum 2^(emin - mant). No overflow handling: the
+/// values rounded here have magnitude <= 1.
+double round_to_format(double x, int mant, int emin) {
+    if (x == 0.0) {
+        return x;
+    }
+    int e = 0;
+    std::frexp(x, &e); // |x| = f 2^e, f in [0.5, 1)
+    int const ex = std::max(e - 1, emin);
+    double const quantum = std::ldexp(1.0, ex - mant);
+    return std::nearbyint(x / quantum) * quantum; // default rounding mode: ties to even
+}
+double round_f16(double x) { return round_to_format(x, 10, -14); }
+double round_bf16(double x) { return round_to_format(x, 7, -126); }
+
+/// IEEE binary16 bit pattern of a value representable in binary16.
+std::uint16_t f16_bits(double v) {
+    std::uint16_t sign = std::signbit(v) ? 0x8000u : 0u;
+    double a = std::fabs(v);
+    if (a == 0.0) {
+        return sign;
+    }
+    if (a < std::ldexp(1.0, -14)) {
+        return static_cast<std::uint16_t>(sign | static_cast<std::uint16_t>(a * std::ldexp(1.0, 24)));
+    }
+    int e = 0;
+    double f = std::frexp(a, &e); // a = f 2^e
+    unsigned const exp = static_cast<unsigned>(e - 1 + 15);
+    unsigned const frac = static_cast<unsigned>(f * 2048.0) - 1024u;
+    return static_cast<std::uint16_t>(sign | (exp << 10) | frac);
+}
+/// bfloat16 bit pattern of a value representable in bfloat16.
+std::uint16_t bf16_bits(double v) {
+    float f = static_cast<float>(v); // exact
+    std::uint32_t b = 0;
+    std::memcpy(&b, &f, sizeof(b));
+    return static_cast<std::uint16_t>(b >> 16);
+}
+
+/// Types and conversions for kernels with half-precision input (emitted when needed).
+std::string half_prelude() {
+    return "#ifndef BBFFT_HIP_HALF_PRELUDE\n#define BBFFT_HIP_HALF_PRELUDE\n"
+           "typedef _Float16 bbfft_h8 __attribute__((ext_vector_type(8)));\n"
+           "typedef __bf16 bbfft_b8 __attribute__((ext_vector_type(8)));\n"
+           "typedef unsigned bbfft_u4 __attribute__((ext_vector_type(4)));\n"
+           "typedef float bbfft_f16v __attribute__((ext_vector_type(16)));\n"
+           "static __device__ __forceinline__ float2 f16c_to_f2(unsigned v) { return "
+           "mk((float)__builtin_bit_cast(_Float16, (unsigned short)(v & 0xffffu)), "
+           "(float)__builtin_bit_cast(_Float16, (unsigned short)(v >> 16))); }\n"
+           "static __device__ __forceinline__ float2 bf16c_to_f2(unsigned v) { return "
+           "mk(__builtin_bit_cast(float, v << 16), __builtin_bit_cast(float, v & 0xffff0000u)); "
+           "}\n"
+           "static __device__ __forceinline__ unsigned ld_nt(const unsigned* p) { return "
+           "__builtin_nontemporal_load(p); }\n"
+           "#endif\n";
+}
 
 /// Line-oriented code writer with fresh-name generation.
 class writer {
@@ -268,9 +330,15 @@ class kernel_gen {
             }
         }
         np_ = (c.lds_pad ? (nfft_ - 1) + (nfft_ - 1) / c.lds_pad : nfft_ - 1) + 1;
+        half_in_ = (c.flags & (fft1d_input_f16 | fft1d_input_bf16)) != 0;
+        bf16_ = (c.flags & fft1d_input_bf16) != 0;
+        mfma_ = (c.flags & fft1d_mfma) != 0;
     }
 
     std::string generate() {
+        if (half_in_) {
+            w_.line(half_prelude());
+        }
         emit_tables();
         emit_kernel();
         return w_.str();
@@ -299,6 +367,9 @@ class kernel_gen {
     }
 
     std::string in_type() const {
+        if (half_in_) {
+            return "unsigned"; // (re, im) half-precision pair
+        }
         return c_.type == transform_type::r2c ? T() : T2();
     }
     std::string out_type() const {
@@ -351,7 +422,43 @@ class kernel_gen {
         w_.line("};");
     }
 
+    std::string mfma_table_name() const { return name_ + "_mA"; }
+
+    /// Terms of the 32 x 32 real matrix of the radix-16 DFT acting on interleaved (re, im)
+    /// vectors: row 2 k1 + co, column 2 n1 + ci, bit patterns in the input's half format.
+    void emit_mfma_table() {
+        unsigned const T = mfma_split_terms(c_.flags);
+        std::vector<std::uint16_t> bits(T * 1024u);
+        for (unsigned i = 0; i < 32; ++i) {
+            for (unsigned q = 0; q < 32; ++q) {
+                unsigned const k1 = i / 2, co = i % 2, n1 = q / 2, ci = q % 2;
+                auto w = gen::root_of_unity(static_cast<long>(c_.direction) * long(n1 * k1), 16);
+                double a = co == 0 ? (ci == 0 ? w.real() : -w.imag())
+                                   : (ci == 0 ? w.imag() : w.real());
+                for (unsigned t = 0; t < T; ++t) {
+                    double h = bf16_ ? round_bf16(a) : round_f16(a);
+                    bits[t * 1024u + i * 32u + q] = bf16_ ? bf16_bits(h) : f16_bits(h);
+                    a -= h; // exact: h is a rounding of a
+                }
+            }
+        }
+        w_.line("static __constant__ __attribute__((aligned(16))) unsigned short " +
+                mfma_table_name() + "[" + std::to_string(bits.size()) + "] = {");
+        std::string row;
+        for (std::size_t i = 0; i < bits.size(); ++i) {
+            row += std::to_string(bits[i]) + "u, ";
+            if ((i + 1) % 16 == 0) {
+                w_.line("  " + row);
+                row.clear();
+            }
+        }
+        w_.line("};");
+    }
+
     void emit_tables() {
+        if (mfma_) {
+            emit_mfma_table();
+        }
         std::size_t p = c_.factorization[0];
         for (std::size_t s = 1; s < L_; ++s) {
             std::size_t R = c_.factorization[s];
@@ -468,11 +575,111 @@ class kernel_gen {
             w_.line("__syncthreads();");
         }
         for (std::size_t s = 0; s < L_; ++s) {
-            emit_stage(s);
+            if (s == 0 && mfma_) {
+                emit_mfma_stage0();
+            } else {
+                emit_stage(s);
+            }
         }
         if (c_.type == transform_type::r2c) {
             emit_r2c_post();
         }
+    }
+
+    /// LDS slot of FFT index n (expression) in the column that starts at pointer `p`.
+    std::string lds_at(std::string const &p, std::string const &n) const {
+        std::string q = c_.lds_pad ? "(" + n + " + ((" + n + ") >> " +
+                                         std::to_string(pad_shift_) + "))"
+                                   : "(" + n + ")";
+        return p + "[" + q + " * " + u(c_.Mb) + "]";
+    }
+
+    /**
+     * Stage 0 (radix 16, p = 1) as a GEMM on the matrix cores. The work-group tile holds
+     * Mb * (N / 16) * Kb columns (m, j, k) of 16 complex inputs x[j + (N / 16) n1]; each wave
+     * takes 32 columns at a time: B (32 x 32, inner index 2 n1 + re/im) is loaded straight from
+     * global memory in the input's half format, A is the split DFT matrix, and
+     * D = sum over terms of A_t B (fp32 accumulation, smallest term first) holds the 16 outputs
+     * of each column, written to LDS slot j * 16 + k1 like the vector stage 0.
+     *
+     * Operand layouts of v_mfma_f32_32x32x16_{f16,bf16} (64 lanes): A row i and B column j
+     * are lane % 32; each lane holds 8 consecutive inner indices, lanes 32..63 the second 8
+     * (any inner-index assignment shared by A and B gives the same product); D element r of
+     * lane l is row 8 (r / 4) + 4 (l / 32) + r % 4 of column l % 32.
+     */
+    void emit_mfma_stage0() {
+        std::size_t const N2 = nfft_ / 16;
+        std::size_t const ncols = c_.Mb * N2 * c_.Kb;
+        std::size_t const nwaves = (c_.work_group_size() + 63) / 64;
+        std::size_t const ntile = (ncols + 31) / 32;
+        std::size_t const iters = (ntile + nwaves - 1) / nwaves;
+        unsigned const T = mfma_split_terms(c_.flags);
+        std::string const vt = bf16_ ? "bbfft_b8" : "bbfft_h8";
+        std::string const op = bf16_ ? "__builtin_amdgcn_mfma_f32_32x32x16_bf16"
+                                     : "__builtin_amdgcn_mfma_f32_32x32x16_f16";
+        w_.line("// stage 0: radix 16 as a DFT-matrix GEMM on MFMA (" + std::to_string(T) +
+                "-term split matrix, fp32 accumulation)");
+        w_.open("");
+        w_.line("const unsigned lane = lid % 64u, wv = lid / 64u;");
+        w_.line("const unsigned hh = lane / 32u, cc = lane % 32u;");
+        for (unsigned t = 0; t < T; ++t) {
+            for (unsigned h2 = 0; h2 < 2; ++h2) {
+                w_.line("const " + vt + " a" + std::to_string(t) + "_" + std::to_string(h2) +
+                        " = __builtin_bit_cast(" + vt + ", *(const bbfft_u4*)(" +
+                        mfma_table_name() + " + " + u(t * 1024 + h2 * 16) +
+                        " + cc * 32u + hh * 8u));");
+            }
+        }
+        if (MG_ == 1) {
+            w_.line("const unsigned long m0 = 0ul;");
+            w_.line("const unsigned long k0 = tile * " + ul(c_.Kb) + ";");
+        } else {
+            w_.line("const unsigned long m0 = (tile % " + ul(MG_) + ") * " + ul(c_.Mb) + ";");
+            w_.line("const unsigned long k0 = (tile / " + ul(MG_) + ") * " + ul(c_.Kb) + ";");
+        }
+        w_.open("for (unsigned it = 0; it < " + u(iters) + "; ++it)");
+        w_.line("const unsigned cl = (wv + " + u(nwaves) + " * it) * 32u + cc;");
+        w_.line("const bool cv = cl < " + u(ncols) + ";");
+        w_.line("const unsigned mm = cl % " + u(c_.Mb) + ";");
+        w_.line("const unsigned jj = (cl / " + u(c_.Mb) + ") % " + u(N2) + ";");
+        w_.line("const unsigned kk = cl / " + u(c_.Mb * N2) + ";");
+        std::string ld = "cv && k0 + kk < kcols";
+        if (MG_ * c_.Mb != c_.M) {
+            ld += " && m0 + mm < " + ul(c_.M);
+        }
+        w_.line("const bool ld = " + ld + ";");
+        w_.line("const unsigned long bb = m0 + mm + (k0 + kk) * " + ul(c_.istride[2]) + ";");
+        for (unsigned h2 = 0; h2 < 2; ++h2) {
+            std::string r = "r" + std::to_string(h2);
+            w_.line("bbfft_u4 " + r + " = {0u, 0u, 0u, 0u};");
+            w_.open("if (ld)");
+            for (unsigned e = 0; e < 4; ++e) {
+                std::string n1 = "(" + u(8 * h2 + e) + " + 4u * hh)";
+                std::string off = "bb + (unsigned long)(jj + " + u(N2) + " * " + n1 + ") * " +
+                                  ul(c_.istride[1]);
+                w_.line(r + "[" + std::to_string(e) + "] = " + gload(off) + ";");
+            }
+            w_.close();
+        }
+        w_.line("bbfft_f16v acc = {};");
+        for (unsigned t = T; t-- > 0;) {
+            for (unsigned h2 = 0; h2 < 2; ++h2) {
+                w_.line("acc = " + op + "(a" + std::to_string(t) + "_" + std::to_string(h2) +
+                        ", __builtin_bit_cast(" + vt + ", r" + std::to_string(h2) +
+                        "), acc, 0, 0, 0);");
+            }
+        }
+        w_.open("if (cv)");
+        w_.line(T2() + "* cp = lds_buf + mm + " + u(c_.Mb * np_) + " * kk;");
+        for (unsigned r = 0; r < 16; r += 2) {
+            std::string k1 = "(" + u(4 * (r / 4) + (r % 4) / 2) + " + 2u * hh)";
+            w_.line(lds_at("cp", "jj * 16u + " + k1) + " = mk(acc[" + std::to_string(r) +
+                    "], acc[" + std::to_string(r + 1) + "]);");
+        }
+        w_.close();
+        w_.close();
+        w_.close();
+        w_.line("__syncthreads();");
     }
 
     /// Offset (elements) of FFT index n in the input column along mode 1.
@@ -497,7 +704,12 @@ class kernel_gen {
     /// Load z[n] of the complex FFT input from global memory into variable v.
     void emit_global_input(std::string const &v, std::string const &n) {
         if (c_.type == transform_type::c2c) {
-            w_.line(v + " = " + gload(ioff(n)) + ";");
+            if (half_in_) {
+                w_.line(v + " = " + std::string(bf16_ ? "bf16c_to_f2(" : "f16c_to_f2(") +
+                        gload(ioff(n)) + ");");
+            } else {
+                w_.line(v + " = " + gload(ioff(n)) + ";");
+            }
         } else if (half_) {
             if (vector_real_in()) {
                 std::string p = "((const " + T2() + "*)(in + ib)) + (" + n + ")";
@@ -749,6 +961,7 @@ class kernel_gen {
     std::size_t nfft_ = 0, L_ = 0, MG_ = 1, np_ = 0;
     unsigned pad_shift_ = 0;
     bool real_ = false, half_ = false, pairs_ = false, use_lds_ = false;
+    bool half_in_ = false, bf16_ = false, mfma_ = false;
 };
 
 } // namespace

@@ -110,6 +110,9 @@ std::shared_ptr<plan_base> create(configuration const &cfg, hipStream_t stream,
 /// Plan of the adjoint of cfg's transform (see adjoint_configuration).
 std::shared_ptr<plan_base> create_adjoint(configuration const &cfg, hipStream_t stream,
                                           jit_cache *cache);
+/// 1-D plan of a kernel variant (fft1d_flag bits, e.g. half-precision input and MFMA stage).
+std::shared_ptr<plan_base> create_variant(configuration const &cfg, unsigned variant,
+                                          hipStream_t stream, jit_cache *cache);
 /// 1-D plan with explicit tuning.
 std::shared_ptr<plan_base> create_tuned(configuration const &cfg, fft1d_tuning const &tuning,
                                         hipStream_t stream, jit_cache *cache);

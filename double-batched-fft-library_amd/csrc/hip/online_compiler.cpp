@@ -70,11 +70,18 @@ void release_module(module_handle_t m) { (void)hipModuleUnload(reinterpret_cast<
 
 } // namespace
 
+/// Options of every kernel compile. Multiply-adds are contracted within a source expression
+/// only, so a kernel's arithmetic depends on its generated expressions and not on the code it
+/// is inlined into (the ragged kernel then computes each job exactly like the plain kernel).
+/// Packed fp32 instructions are disabled.
+static std::vector<std::string> base_options(std::string const &arch) {
+    return {"--offload-arch=" + arch, "-O3", "-std=c++17", "-ffp-contract=on", "-Xclang",
+            "-target-feature", "-Xclang", "-packed-fp32-ops"};
+}
+
 std::vector<std::uint8_t> compile_to_native(std::string const &source, std::string const &arch,
                                             std::vector<std::string> const &options) {
-    std::vector<std::string> opts = {"--offload-arch=" + arch, "-O3", "-std=c++17",
-                                     "-ffp-contract=fast", "-Xclang", "-target-feature",
-                                     "-Xclang", "-packed-fp32-ops"};
+    std::vector<std::string> opts = base_options(arch);
     opts.insert(opts.end(), options.begin(), options.end());
     std::vector<char const *> copts;
     for (auto const &o : opts) {
@@ -126,6 +133,9 @@ hipModule_t build_kernel_bundle(std::string const &source, int device,
         int major = 0, minor = 0;
         hiprtcVersion(&major, &minor);
         std::string key = arch + '\n' + std::to_string(major) + '.' + std::to_string(minor);
+        for (auto const &o : base_options(arch)) {
+            key += '\n' + o;
+        }
         for (auto const &o : options) {
             key += '\n' + o;
         }

@@ -60,6 +60,9 @@ fft1d_plan::fft1d_plan(configuration const &cfg, hipStream_t stream, int device,
     std::size_t const rb = static_cast<std::size_t>(kc_.fp);
     // Real data read or written as (x[2n], x[2n+1]) pairs needs complex alignment.
     in_align_ = kc_.type == transform_type::r2c ? rb : 2 * rb;
+    if (kc_.flags & (fft1d_input_f16 | fft1d_input_bf16)) {
+        in_align_ = 4; // (re, im) half pairs
+    }
     out_align_ = kc_.type == transform_type::c2r ? rb : 2 * rb;
     if (kc_.N % 2 == 0 && kc_.M == 1 && kc_.type == transform_type::r2c &&
         !kc_.load_function && kc_.istride[1] == 1 && kc_.istride[2] % 2 == 0) {
@@ -182,6 +185,18 @@ std::shared_ptr<plan_base> create_adjoint(configuration const &cfg, hipStream_t 
                                             fft1d_tuning{}, variant);
     }
     return std::make_shared<nd_plan>(layout, stream, device, cache, variant);
+}
+
+std::shared_ptr<plan_base> create_variant(configuration const &cfg, unsigned variant,
+                                          hipStream_t stream, jit_cache *cache) {
+    auto layout = decompose(cfg);
+    if (layout.passes.size() != 1) {
+        throw bad_configuration("Kernel variants apply to 1-D configurations only");
+    }
+    int device = 0;
+    BBFFT_HIP_CHECK(hipGetDevice(&device));
+    return std::make_shared<fft1d_plan>(layout.passes[0], stream, device, cache, fft1d_tuning{},
+                                        variant);
 }
 
 std::shared_ptr<plan_base> create_tuned(configuration const &cfg, fft1d_tuning const &tuning,

@@ -71,7 +71,20 @@ enum fft1d_flag : unsigned {
     /// imag(X[N/2]) = -real(X[0]) in the post-pass, making it the adjoint of c2r (including the
     /// c2r Nyquist rule).
     fft1d_adjoint = 4u,
+    /// c2c fp32 only: the input is stored as IEEE half-precision complex (re, im) pairs (4 bytes
+    /// per element, real part in the low half); the output stays fp32 complex.
+    fft1d_input_f16 = 8u,
+    /// As fft1d_input_f16 with bfloat16 (re, im) pairs.
+    fft1d_input_bf16 = 16u,
+    /// With fft1d_input_f16 / fft1d_input_bf16: the first stage (radix 16) runs as a DFT-matrix
+    /// GEMM on the matrix cores (v_mfma_f32_32x32x16_{f16,bf16}) with fp32 accumulation; the
+    /// DFT matrix is split into 2 (f16) or 3 (bf16) half-precision terms whose products with the
+    /// exactly representable input are accumulated, so the stage is accurate to fp32 level.
+    fft1d_mfma = 32u,
 };
+
+/// Number of half-precision terms the DFT matrix is split into on the MFMA stage.
+inline unsigned mfma_split_terms(unsigned flags) { return (flags & fft1d_input_bf16) ? 3u : 2u; }
 
 /// Optional overrides of the tiling heuristics (0 / empty = choose automatically).
 struct BBFFT_EXPORT fft1d_tuning {

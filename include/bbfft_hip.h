@@ -180,6 +180,20 @@ BBFFT_EXPORT_C void bbfft_plan_destroy(bbfft_plan_t plan);
 BBFFT_EXPORT_C int bbfft_device_info_query(int device, char *buf, size_t len);
 
 /*
+ * Half-precision input (extension; no reference counterpart). For a 1-D c2c fp32
+ * configuration, the input holds (re, im) pairs of IEEE binary16 (BBFFT_INPUT_F16) or bfloat16
+ * (BBFFT_INPUT_BF16) with the configuration's strides (counted in complex elements, 4 bytes
+ * each); the output is fp32 complex. With BBFFT_MFMA the first stage (radix 16; N must be a
+ * multiple of 16, N >= 32) runs as a DFT-matrix GEMM on the matrix cores with the matrix split
+ * into 2 (f16) or 3 (bf16) half-precision terms and fp32 accumulation.
+ */
+enum { BBFFT_INPUT_F16 = 8, BBFFT_INPUT_BF16 = 16, BBFFT_MFMA = 32 };
+
+BBFFT_EXPORT_C int bbfft_plan_create_half_input(const bbfft_configuration *cfg, unsigned flags,
+                                                bbfft_stream_t stream, bbfft_cache_t cache,
+                                                bbfft_plan_t *plan);
+
+/*
  * Ragged batches (extension; no reference counterpart: the reference runs one plan per shape,
  * src/common/algorithm/factor2_slm_fft.hpp:92-203). A job is a c2c M x N x K tensor with packed
  * column-major strides {1, M, M*N}; in_offset / out_offset count complex elements from the

@@ -116,4 +116,37 @@ template <typename Body> int launch(unsigned grid, unsigned block, Body body) {
 
 } // namespace emu
 
+#if defined(__clang__)
+namespace emu {
+inline float mfma_a[16][64][8];
+inline float mfma_b[16][64][8];
+
+/// One v_mfma_f32_32x32x16_{f16,bf16} of a wave: A row / B column = lane % 32, inner index
+/// 8 (lane / 32) + item; D item r of lane l = row 8 (r / 4) + 4 (l / 32) + r % 4, column
+/// l % 32. Every lane of the work-group must call it (two work-group barriers).
+template <typename V8, typename F16> F16 mfma_32x32x16(V8 a, V8 b, F16 c) {
+    unsigned const lid = threadIdx_.x, w = lid / 64u, l = lid % 64u;
+    for (int e = 0; e < 8; ++e) {
+        mfma_a[w][l][e] = static_cast<float>(a[e]);
+        mfma_b[w][l][e] = static_cast<float>(b[e]);
+    }
+    __syncthreads();
+    F16 d = c;
+    unsigned const j = l % 32u;
+    for (unsigned r = 0; r < 16; ++r) {
+        unsigned const i = 8u * (r / 4u) + 4u * (l / 32u) + r % 4u;
+        float s = d[r];
+        for (unsigned k = 0; k < 16; ++k) {
+            s += mfma_a[w][i + 32u * (k / 8u)][k % 8u] * mfma_b[w][j + 32u * (k / 8u)][k % 8u];
+        }
+        d[r] = s;
+    }
+    __syncthreads();
+    return d;
+}
+} // namespace emu
+#define __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, x, y, z) emu::mfma_32x32x16(a, b, c)
+#define __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, x, y, z) emu::mfma_32x32x16(a, b, c)
+#endif
+
 #endif

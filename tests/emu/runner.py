@@ -12,6 +12,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _BUILD = os.path.join(_HERE, "_build")
+CLANG = "/opt/rocm/llvm/bin/clang++"
 
 
 def _build_so(tu):
@@ -23,8 +24,10 @@ def _build_so(tu):
         with open(src, "w") as f:
             f.write(tu)
         tmp = so + f".tmp{os.getpid()}"
-        subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-w", "-I", _HERE, src,
-                        "-o", tmp], check=True)
+        # Kernels with half-precision types need clang (_Float16, __bf16, ext_vector_type).
+        cxx = CLANG if "BBFFT_HIP_HALF_PRELUDE" in tu else "g++"
+        subprocess.run([cxx, "-O1", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-w",
+                        "-I", _HERE, src, "-o", tmp], check=True)
         os.replace(tmp, so)
     return ctypes.CDLL(so)
 
@@ -72,6 +75,26 @@ def run(bbfft, cfg, x, out_size, grid_cap=None, device_info=None, pass_index=0,
         return in_place_buffer
     xin = np.ascontiguousarray(x, dtype=npr if t == 1 else npc)
     out = np.zeros(out_size, dtype=npr if t == 2 else npc)
+    rc = fn(xin.ctypes.data, out.ctypes.data, K, grid, kc["work_group_size"])
+    if rc != 0:
+        raise RuntimeError("divergent barrier in emulated kernel")
+    return out
+
+
+def run_half_input(bbfft, cfg, x_bits, flags, grid_cap=None):
+    """Execute a 1-D c2c fp32 kernel reading half-precision (re, im) pairs (x_bits: uint32
+    array, real part in the low 16 bits) with fft1d flags `flags`; returns complex64 output."""
+    info = bbfft.GFX950_INFO
+    tuning = {"flags": flags}
+    kc = bbfft.kernel_configuration(cfg, 0, info, tuning=tuning)
+    src = bbfft.generate_source(cfg, 0, info, tuning=tuning)
+    fn = compile_kernel(src, kc["name"], "unsigned", "float2")
+    M, N, K = kc["M"], kc["N"], kc["K"]
+    ntiles = -(-M // kc["Mb"]) * (-(-K // kc["Kb"]))
+    grid = ntiles if grid_cap is None else min(ntiles, grid_cap)
+    xin = np.ascontiguousarray(x_bits, dtype=np.uint32)
+    n_out = sum((s - 1) * st for s, st in zip([M, N, K], cfg.ostride[:3])) + 1
+    out = np.zeros(n_out, dtype=np.complex64)
     rc = fn(xin.ctypes.data, out.ctypes.data, K, grid, kc["work_group_size"])
     if rc != 0:
         raise RuntimeError("divergent barrier in emulated kernel")
