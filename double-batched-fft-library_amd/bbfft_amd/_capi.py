@@ -167,7 +167,7 @@ def load():
                                                        c_size, P(CKernelConfig)]),
         "bbfft_ragged_job_variants": (c_int, [c_int, P(CRaggedJob), c_size, c_char_p,
                                               P(c_uint)]),
-        "bbfft_ragged_job_table": (c_int, [c_int, P(CRaggedJob), c_size, c_char_p,
+        "bbfft_ragged_job_table": (c_int, [c_int, P(CRaggedJob), c_size, c_char_p, c_size,
                                            P(ctypes.c_ulonglong), c_size, P(c_size)]),
         "bbfft_generate_ragged_source": (c_int, [c_int, P(CRaggedJob), c_size, c_char_p,
                                                  c_char_p, c_size, P(c_size)]),

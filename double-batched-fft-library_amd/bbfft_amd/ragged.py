@@ -108,14 +108,15 @@ def job_variants(jobs, precision=4, device_info=GFX950_INFO) -> List[int]:
     return list(out)[:len(jobs)]
 
 
-def job_table(jobs, precision=4, device_info=GFX950_INFO) -> np.ndarray:
+def job_table(jobs, precision=4, device_info=GFX950_INFO, grid=1) -> np.ndarray:
+    """Device job table for a launch of `grid` work-groups (see bbfft_ragged_job_table)."""
     L = _capi.load()
     cj = _c_jobs(jobs)
     n = ctypes.c_size_t()
-    _check(L.bbfft_ragged_job_table(precision, cj, len(jobs), device_info.encode(), None, 0,
-                                    ctypes.byref(n)))
+    _check(L.bbfft_ragged_job_table(precision, cj, len(jobs), device_info.encode(), grid, None,
+                                    0, ctypes.byref(n)))
     t = np.zeros(n.value, dtype=np.uint64)
-    _check(L.bbfft_ragged_job_table(precision, cj, len(jobs), device_info.encode(),
+    _check(L.bbfft_ragged_job_table(precision, cj, len(jobs), device_info.encode(), grid,
                                     t.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong)),
                                     n.value, ctypes.byref(n)))
     return t

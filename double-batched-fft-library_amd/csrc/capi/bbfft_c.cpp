@@ -542,12 +542,13 @@ int bbfft_ragged_job_variants(int prec, const bbfft_ragged_job *jobs, size_t num
 }
 
 int bbfft_ragged_job_table(int prec, const bbfft_ragged_job *jobs, size_t num_jobs,
-                           const char *device_info, unsigned long long *table, size_t len,
-                           size_t *needed) {
+                           const char *device_info, size_t grid, unsigned long long *table,
+                           size_t len, size_t *needed) {
     return guarded([&] {
         require(device_info != nullptr, "null argument");
+        require(grid > 0, "grid must be positive");
         auto r = ragged_of(prec, jobs, num_jobs);
-        auto t = ragged_job_table(r, configure_ragged(r, parse_device_info(device_info)));
+        auto t = ragged_job_table(r, configure_ragged(r, parse_device_info(device_info)), grid);
         if (needed) {
             *needed = t.size();
         }

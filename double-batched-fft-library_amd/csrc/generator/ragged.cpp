@@ -9,6 +9,7 @@
 #include "heuristics.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <iomanip>
 #include <map>
@@ -171,20 +172,57 @@ ragged_kernel_configuration configure_ragged(ragged_configuration const &cfg,
 }
 
 std::vector<std::uint64_t> ragged_job_table(ragged_configuration const &cfg,
-                                            ragged_kernel_configuration const &rk) {
+                                            ragged_kernel_configuration const &rk,
+                                            std::size_t grid) {
     std::size_t const n = cfg.jobs.size();
     if (rk.order.size() != n || rk.tile_begin.size() != n + 1) {
         throw bad_configuration("Ragged kernel configuration does not match the jobs");
     }
-    std::vector<std::uint64_t> table(n + 1 + 4 * n);
+    if (grid == 0) {
+        throw bad_configuration("The work split needs at least one work-group");
+    }
+    std::vector<std::uint64_t> table(n + 1 + 4 * n + (grid + 1) + grid);
     std::copy(rk.tile_begin.begin(), rk.tile_begin.end(), table.begin());
+    std::vector<double> weight(n); // elements per tile of each table position
     for (std::size_t p = 0; p < n; ++p) {
         auto const &job = cfg.jobs[rk.order[p]];
+        auto const &v = rk.variants[rk.job_variant[rk.order[p]]];
         std::uint64_t *r = table.data() + n + 1 + 4 * p;
         r[0] = job.in_offset;
         r[1] = job.out_offset;
         r[2] = job.K;
         r[3] = rk.job_variant[rk.order[p]];
+        weight[p] = static_cast<double>(v.Mb * v.Kb * v.N);
+    }
+    // Work split: boundary g sits at g / grid of the total weight.
+    std::vector<double> cum(n + 1, 0.0);
+    for (std::size_t p = 0; p < n; ++p) {
+        cum[p + 1] = cum[p] + weight[p] * static_cast<double>(rk.tile_begin[p + 1] -
+                                                              rk.tile_begin[p]);
+    }
+    std::uint64_t *first = table.data() + 5 * n + 1;
+    std::uint64_t *pos = first + grid + 1;
+    std::size_t p = 0;
+    for (std::size_t g = 0; g <= grid; ++g) {
+        std::uint64_t t = rk.total_tiles();
+        if (g < grid) {
+            double const target = cum[n] * static_cast<double>(g) / static_cast<double>(grid);
+            while (p + 1 < n && cum[p + 1] <= target) {
+                ++p;
+            }
+            double const k = std::ceil((target - cum[p]) / weight[p]);
+            t = std::min<std::uint64_t>(rk.tile_begin[p] + static_cast<std::uint64_t>(
+                                                               std::max(0.0, k)),
+                                        rk.tile_begin[p + 1]);
+            t = std::max<std::uint64_t>(t, g ? first[g - 1] : 0);
+        }
+        first[g] = t;
+    }
+    // job holding each first tile: the last position whose first tile is <= it
+    for (std::size_t g = 0; g < grid; ++g) {
+        auto it = std::upper_bound(rk.tile_begin.begin(), rk.tile_begin.end() - 1, first[g]);
+        pos[g] = static_cast<std::uint64_t>(std::max<std::ptrdiff_t>(
+            0, (it - rk.tile_begin.begin()) - 1));
     }
     return table;
 }
@@ -205,15 +243,18 @@ void generate_ragged(std::ostream &os, ragged_kernel_configuration const &rk,
        << "(const " << T2 << "* in, " << T2
        << "* out, const unsigned long* table, unsigned long njobs) {\n"
        << "  __shared__ " << T2 << " lds_buf[" << lds_elems << "];\n"
-       << "  const unsigned long total = table[njobs];\n"
-       << "  for (unsigned long tile = blockIdx.x; tile < total; tile += gridDim.x) {\n"
-       << "    unsigned long lo = 0ul, hi = njobs;\n"
-       << "    while (hi - lo > 1ul) {\n"
-       << "      const unsigned long mid = (lo + hi) / 2ul;\n"
-       << "      if (table[mid] <= tile) { lo = mid; } else { hi = mid; }\n"
+       << "  const unsigned long* first = table + 5ul * njobs + 1ul;\n"
+       << "  const unsigned long* pos = first + gridDim.x + 1ul;\n"
+       << "  const unsigned long tend = first[blockIdx.x + 1u];\n"
+       << "  unsigned long p = pos[blockIdx.x];\n"
+       << "  unsigned long b = table[p], nb = table[p + 1ul];\n"
+       << "  const unsigned long* rec = table + njobs + 1ul + 4ul * p;\n"
+       << "  for (unsigned long tile = first[blockIdx.x]; tile < tend; ++tile) {\n"
+       << "    if (tile >= nb) {\n"
+       << "      do { ++p; b = nb; nb = table[p + 1ul]; } while (tile >= nb);\n"
+       << "      rec = table + njobs + 1ul + 4ul * p;\n"
        << "    }\n"
-       << "    const unsigned long* rec = table + njobs + 1ul + 4ul * lo;\n"
-       << "    const unsigned long lt = tile - table[lo];\n"
+       << "    const unsigned long lt = tile - b;\n"
        << "    const " << T2 << "* jin = in + rec[0];\n"
        << "    " << T2 << "* jout = out + rec[1];\n"
        << "    const unsigned long K = rec[2];\n"

@@ -64,7 +64,7 @@ ragged_plan::ragged_plan(ragged_configuration const &cfg, hipStream_t stream, in
         in_end_ = std::max(in_end_, job.in_offset + n);
         out_end_ = std::max(out_end_, job.out_offset + n);
     }
-    auto table = ragged_job_table(cfg, rk_);
+    auto table = ragged_job_table(cfg, rk_, grid_);
     njobs_ = cfg.jobs.size();
     BBFFT_HIP_CHECK(hipMalloc(&table_, table.size() * sizeof(std::uint64_t)));
     BBFFT_HIP_CHECK(hipMemcpy(table_, table.data(), table.size() * sizeof(std::uint64_t),

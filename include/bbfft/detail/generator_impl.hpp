@@ -147,12 +147,16 @@ BBFFT_EXPORT ragged_kernel_configuration configure_ragged(ragged_configuration c
 BBFFT_EXPORT void generate_ragged(std::ostream &os, ragged_kernel_configuration const &rk,
                                   std::string_view name = {});
 /**
- * @brief Device job table: tile_begin[0..njobs] (njobs + 1 entries, the last is the total),
- * followed by njobs records {in_offset, out_offset, K, variant} in table order (offsets in
- * complex elements).
+ * @brief Device job table for a launch of `grid` work-groups: tile_begin[0..njobs] (njobs + 1
+ * entries, the last is the total), njobs records {in_offset, out_offset, K, variant} in table
+ * order (offsets in complex elements), then the work split: first tile of each work-group
+ * (grid + 1 entries, the last is the total) and the table position of the job holding that
+ * tile (grid entries). Work-group g processes the contiguous tiles [first[g], first[g + 1]);
+ * the boundaries balance the elements processed (tile count times elements per tile).
  */
 BBFFT_EXPORT std::vector<std::uint64_t> ragged_job_table(ragged_configuration const &cfg,
-                                                         ragged_kernel_configuration const &rk);
+                                                         ragged_kernel_configuration const &rk,
+                                                         std::size_t grid);
 
 } // namespace bbfft
 

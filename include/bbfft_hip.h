@@ -227,9 +227,10 @@ BBFFT_EXPORT_C int bbfft_ragged_variant_configuration(int precision,
 BBFFT_EXPORT_C int bbfft_ragged_job_variants(int precision, const bbfft_ragged_job *jobs,
                                              size_t num_jobs, const char *device_info,
                                              unsigned *job_variant);
-/* Device job table (num_jobs + 1 + 4 * num_jobs 64-bit words), as the kernel reads it. */
+/* Device job table for a launch of `grid` work-groups (5 * num_jobs + 2 * grid + 2 64-bit
+ * words: tile starts, job records, per-work-group tile ranges), as the kernel reads it. */
 BBFFT_EXPORT_C int bbfft_ragged_job_table(int precision, const bbfft_ragged_job *jobs,
-                                          size_t num_jobs, const char *device_info,
+                                          size_t num_jobs, const char *device_info, size_t grid,
                                           unsigned long long *table, size_t len, size_t *needed);
 BBFFT_EXPORT_C int bbfft_generate_ragged_source(int precision, const bbfft_ragged_job *jobs,
                                                 size_t num_jobs, const char *device_info,

@@ -105,7 +105,7 @@ def run_ragged(ragged, jobs, x, out_size, precision=4, grid=7, in_place=False):
     """Execute the persistent ragged kernel for `jobs` on host arrays (`grid` work-groups)."""
     kc = ragged.kernel_configuration(jobs, precision)
     src = ragged.generate_source(jobs, precision)
-    table = ragged.job_table(jobs, precision)
+    table = ragged.job_table(jobs, precision, grid=grid)
     c2 = "float2" if precision == 4 else "double2"
     npc = np.complex64 if precision == 4 else np.complex128
     name = kc["name"]

@@ -91,27 +91,53 @@ def test_job_table_layout():
     kc = ragged.kernel_configuration(jobs)
     var = ragged.job_variants(jobs)
     assert var[0] == var[2] and len(set(var)) == 3 and kc["num_variants"] == 3
-    t = ragged.job_table(jobs)
+    grid = 5
+    t = ragged.job_table(jobs, grid=grid)
     nj = len(jobs)
-    assert len(t) == nj + 1 + 4 * nj
+    assert len(t) == 5 * nj + 1 + 2 * grid + 1
     # table order groups jobs by variant, keeping the given order within a variant
-    recs = t[nj + 1:].reshape(nj, 4)
+    recs = t[nj + 1:5 * nj + 1].reshape(nj, 4)
     order = [next(i for i, j in enumerate(jobs) if j.in_offset == r[0]) for r in recs]
     assert order == sorted(range(nj), key=lambda i: (var[i], i))
     # tile counts: ceil(M / Mb) * ceil(K / Kb) per job, prefix sums
-    tiles = []
+    tiles, weights = [], []
     for p, i in enumerate(order):
         vc = ragged.variant_configuration(jobs, var[i])
         tiles.append(-(-jobs[i].M // vc["Mb"]) * -(-jobs[i].K // vc["Kb"]))
+        weights.append(vc["Mb"] * vc["Kb"] * jobs[i].N)
         assert recs[p][1] == jobs[i].out_offset and recs[p][2] == jobs[i].K
         assert recs[p][3] == var[i]
-    assert list(t[:nj + 1]) == [0] + list(np.cumsum(tiles))
+    begin = [0] + list(np.cumsum(tiles))
+    assert list(t[:nj + 1]) == begin
     assert kc["total_tiles"] == sum(tiles)
     for v in range(kc["num_variants"]):
         vc = ragged.variant_configuration(jobs, v)
         assert vc["work_group_size"] <= kc["group_size"]
         assert max(vc["factorization"]) <= 16
         assert vc["lds_bytes"] <= kc["lds_bytes"]
+    # work split: contiguous, covering, balanced by elements, with the job of each first tile
+    first = t[5 * nj + 1:5 * nj + 2 + grid]
+    pos = t[5 * nj + 2 + grid:]
+    assert first[0] == 0 and first[-1] == sum(tiles) and np.all(np.diff(first.astype(np.int64)) >= 0)
+    tile_w = np.repeat(weights, tiles)
+    per_wg = [tile_w[first[g]:first[g + 1]].sum() for g in range(grid)]
+    ideal = sum(per_wg) / grid  # boundaries are rounded to whole tiles
+    assert all(abs(w - ideal) <= max(weights) for w in per_wg)
+    for g in range(grid):
+        p = int(pos[g])
+        assert begin[p] <= first[g] and (first[g] < begin[p + 1] or first[g] == first[g + 1])
+
+
+@pytest.mark.parametrize("grid", [1, 2, 7, 64])
+def test_work_split_grids(grid):
+    """Any grid size gives the same results (every tile processed exactly once)."""
+    jobs = _mixed_jobs(9, 7, (12, 60, 128, 243), kmax=12)
+    n = ragged.total_elements(jobs)
+    x = oracle.uniform_complex(9, n)
+    y = runner.run_ragged(ragged, jobs, x, n, grid=grid)
+    for j in jobs:
+        assert oracle.rel_l2(y[j.out_offset:j.out_offset + j.elements],
+                             _ref(j, x, 4)) <= oracle.tolerance(4, j.N)
 
 
 def test_identifier_depends_on_variants_only():
