@@ -409,6 +409,33 @@ def make_adjoint_plan(cfg, stream=None, cache: Optional["Cache"] = None) -> "Pla
     return Plan(h, adjoint_configuration(cfg), stream)
 
 
+def fused_configuration(cfg: configuration, device_info=GFX950_INFO):
+    """The fused pair of passes of a 2-D/3-D configuration on a device (None if not fused)."""
+    fused = ctypes.c_int()
+    info = _capi.CFusedInfo()
+    name = ctypes.create_string_buffer(256)
+    _check(_capi.load().bbfft_fused_configuration(ctypes.byref(cfg._to_c()),
+                                                  device_info.encode(), ctypes.byref(fused),
+                                                  ctypes.byref(info), name, 256))
+    if not fused.value:
+        return None
+    out = {k: getattr(info, k) for k, _ in _capi.CFusedInfo._fields_}
+    out["name"] = name.value.decode()
+    return out
+
+
+def generate_fused_source(cfg: configuration, device_info=GFX950_INFO) -> str:
+    L = _capi.load()
+    n = ctypes.c_size_t()
+    c = cfg._to_c()
+    _check(L.bbfft_generate_fused_source(ctypes.byref(c), device_info.encode(), None, 0,
+                                         ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value)
+    _check(L.bbfft_generate_fused_source(ctypes.byref(c), device_info.encode(), buf, n.value,
+                                         ctypes.byref(n)))
+    return buf.value.decode()
+
+
 INPUT_F16 = _capi.INPUT_F16
 INPUT_BF16 = _capi.INPUT_BF16
 MFMA = _capi.MFMA

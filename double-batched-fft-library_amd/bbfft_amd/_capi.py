@@ -26,6 +26,7 @@ EXPORTED = [
     "bbfft_ragged_kernel_configuration", "bbfft_ragged_variant_configuration",
     "bbfft_ragged_job_variants", "bbfft_ragged_job_table", "bbfft_generate_ragged_source",
     "bbfft_ragged_plan_create", "bbfft_aot_generate_ragged", "bbfft_plan_create_half_input",
+    "bbfft_fused_configuration", "bbfft_generate_fused_source",
 ]
 
 SUCCESS = 0
@@ -87,6 +88,15 @@ class CRaggedInfo(ctypes.Structure):
     _fields_ = [
         ("group_size", ctypes.c_size_t), ("lds_bytes", ctypes.c_size_t),
         ("num_variants", ctypes.c_size_t), ("total_tiles", ctypes.c_ulonglong),
+    ]
+
+
+class CFusedInfo(ctypes.Structure):
+    _fields_ = [
+        ("first_pass", ctypes.c_uint), ("group_size", ctypes.c_size_t),
+        ("lds_bytes", ctypes.c_size_t), ("planes", ctypes.c_size_t), ("ka", ctypes.c_size_t),
+        ("kb", ctypes.c_size_t), ("plane_elems", ctypes.c_size_t),
+        ("in_plane_stride", ctypes.c_size_t), ("out_plane_stride", ctypes.c_size_t),
     ]
 
 
@@ -177,6 +187,10 @@ def load():
                                               c_char_p]),
         "bbfft_plan_create_half_input": (c_int, [P(CConfiguration), c_uint, c_void_p, c_void_p,
                                                  P(c_void_p)]),
+        "bbfft_fused_configuration": (c_int, [P(CConfiguration), c_char_p, P(c_int),
+                                              P(CFusedInfo), c_char_p, c_size]),
+        "bbfft_generate_fused_source": (c_int, [P(CConfiguration), c_char_p, c_char_p, c_size,
+                                                P(c_size)]),
     }
     for name, (res, args) in protos.items():
         fn = getattr(L, name)

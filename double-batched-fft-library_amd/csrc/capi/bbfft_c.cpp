@@ -488,6 +488,50 @@ int bbfft_device_info_query(int device, char *buf, size_t len) {
     return guarded([&] { copy_out(hip::get_device_info(device).to_string(), buf, len); });
 }
 
+int bbfft_fused_configuration(const bbfft_configuration *cfg, const char *device_info,
+                              int *fused, bbfft_fused_info *info, char *name, size_t name_len) {
+    return guarded([&] {
+        require(cfg && device_info && fused, "null argument");
+        auto layout = decompose(from_c(*cfg));
+        fused_pair fp;
+        *fused = find_fused_pair(layout, parse_device_info(device_info), 0u, fp) ? 1 : 0;
+        if (*fused && info) {
+            info->first_pass = static_cast<unsigned>(fp.first);
+            info->group_size = fp.fc.group_size;
+            info->lds_bytes = fp.fc.lds_bytes();
+            info->planes = fp.fc.planes;
+            info->ka = fp.fc.ka;
+            info->kb = fp.fc.kb;
+            info->plane_elems = fp.fc.plane_elems;
+            info->in_plane_stride = fp.fc.in_plane_stride;
+            info->out_plane_stride = fp.fc.out_plane_stride;
+        }
+        if (*fused && name) {
+            copy_out(fp.fc.identifier(), name, name_len);
+        }
+    });
+}
+
+int bbfft_generate_fused_source(const bbfft_configuration *cfg, const char *device_info,
+                                char *buf, size_t len, size_t *needed) {
+    return guarded([&] {
+        require(cfg && device_info, "null argument");
+        auto layout = decompose(from_c(*cfg));
+        fused_pair fp;
+        require(find_fused_pair(layout, parse_device_info(device_info), 0u, fp),
+                "the configuration has no fused passes on this device");
+        std::ostringstream os;
+        generate_fused2(os, fp.fc);
+        auto str = os.str();
+        if (needed) {
+            *needed = str.size() + 1;
+        }
+        if (buf) {
+            copy_out(str, buf, len);
+        }
+    });
+}
+
 int bbfft_plan_create_half_input(const bbfft_configuration *cfg, unsigned flags,
                                  bbfft_stream_t stream, bbfft_cache_t cache, bbfft_plan_t *plan) {
     return guarded([&] {
@@ -636,8 +680,23 @@ int bbfft_aot_generate(const char *const *descriptors, size_t num_descriptors,
         std::size_t index = 0;
         for (size_t i = 0; i < num_descriptors; ++i) {
             auto cfg = parse_fft_descriptor(descriptors[i]);
-            for (auto const &p : decompose(cfg).passes) {
-                auto kc = configure_fft1d(p, info);
+            auto layout = decompose(cfg);
+            // the kernels a plan for this descriptor uses: the fused pair (if any) and the
+            // remaining passes
+            fused_pair fp;
+            bool const fused = find_fused_pair(layout, info, 0u, fp);
+            if (fused && seen.insert(fp.fc.identifier()).second) {
+                if (index++ % num_shards == shard) {
+                    std::ostringstream os;
+                    generate_fused2(os, fp.fc);
+                    kernels.emplace_back(fp.fc.identifier(), os.str());
+                }
+            }
+            for (std::size_t q = 0; q < layout.passes.size(); ++q) {
+                if (fused && (q == fp.first || q == fp.first + 1)) {
+                    continue;
+                }
+                auto kc = configure_fft1d(layout.passes[q], info);
                 auto id = kc.identifier();
                 if (seen.insert(id).second) {
                     if (index++ % num_shards == shard) {

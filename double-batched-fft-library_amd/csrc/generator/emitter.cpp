@@ -314,8 +314,13 @@ std::string ul(std::size_t v) { return std::to_string(v) + "ul"; }
 /// Emits one kernel.
 class kernel_gen {
   public:
-    kernel_gen(fft1d_configuration const &c, std::string name, std::size_t group_size = 0)
-        : c_(c), f_{c.fp}, name_(std::move(name)), group_size_(group_size) {
+    /// alias_in / alias_out: the input / output pointer of a tile body points into the
+    /// work-group's LDS scratch (fused multi-dimensional kernels), so reads of the input and
+    /// writes of the output are separated from scratch writes / reads by barriers.
+    kernel_gen(fft1d_configuration const &c, std::string name, std::size_t group_size = 0,
+               bool alias_in = false, bool alias_out = false)
+        : c_(c), f_{c.fp}, name_(std::move(name)), group_size_(group_size), alias_in_(alias_in),
+          alias_out_(alias_out) {
         nfft_ = c.N_fft();
         L_ = c.factorization.size();
         real_ = c.type != transform_type::c2c;
@@ -755,7 +760,10 @@ class kernel_gen {
         std::size_t const B = (NR + c_.Nt - 1) / c_.Nt;
         bool const src_global = s == 0 && c_.type != transform_type::c2r;
         bool const dst_global = s + 1 == L_ && c_.type != transform_type::r2c;
-        bool const mid_barrier = !src_global && !dst_global;
+        // Reads and writes of LDS (or of memory aliasing it) are separated by a barrier.
+        bool const reads_shared = !src_global || alias_in_;
+        bool const writes_shared = !dst_global || alias_out_;
+        bool const mid_barrier = reads_shared && writes_shared;
 
         w_.line("// stage " + std::to_string(s) + ": radix " + std::to_string(R) + ", p = " +
                 std::to_string(p));
@@ -855,9 +863,12 @@ class kernel_gen {
         std::size_t const H = nfft_;
         std::size_t const Q = half_ ? H / 2 + 1 : (c_.N - 1) / 2 + 1;
         std::size_t const B = (Q + c_.Nt - 1) / c_.Nt;
+        // phase 1: read the half spectrum and form the complex FFT inputs
         for (std::size_t b = 0; b < B; ++b) {
             std::string q = "q" + std::to_string(b);
+            std::string s1 = "s1_" + std::to_string(b), s2 = "s2_" + std::to_string(b);
             w_.line("const unsigned " + q + " = t + " + u(b * c_.Nt) + ";");
+            w_.line(T2() + " " + s1 + " = " + f_.zero2() + ", " + s2 + " = " + f_.zero2() + ";");
             w_.open("if (active && " + q + " < " + u(Q) + ")");
             if (half_) {
                 w_.line(T2() + " x1 = " + gload(ioff(q)) + ";");
@@ -873,9 +884,8 @@ class kernel_gen {
                 w_.line("const " + T2() + " a = c_add(x1, x2);");
                 w_.line("const " + T2() + " bt = c_mul(c_sub(x1, x2), " + twr_name() + "[" + q +
                         "]);");
-                w_.line(lds(q + " % " + u(H)) + " = c_add(a, bt);");
-                w_.line("if (" + q + " != 0u && 2u * " + q + " != " + u(H) + ") " +
-                        lds(u(H) + " - " + q) + " = mk(a.x - bt.x, bt.y - a.y);");
+                w_.line(s1 + " = c_add(a, bt);");
+                w_.line(s2 + " = mk(a.x - bt.x, bt.y - a.y);");
             } else {
                 w_.line(T2() + " ai = " + gload(ioff(q)) + ";");
                 w_.line(T2() + " bi = " + f_.zero2() + ";");
@@ -887,9 +897,27 @@ class kernel_gen {
                             "); bi = c_scale(bi, " + f_.lit(0.5) + "); }");
                 }
                 w_.line("const " + T2() + " ib2 = c_ipos(bi);");
-                w_.line(lds(q) + " = c_add(ai, ib2);");
-                w_.line("if (" + q + " != 0u) { const " + T2() + " d = c_sub(ai, ib2); " +
-                        lds(u(c_.N) + " - " + q) + " = mk(d.x, -d.y); }");
+                w_.line(s1 + " = c_add(ai, ib2);");
+                w_.line("const " + T2() + " d = c_sub(ai, ib2);");
+                w_.line(s2 + " = mk(d.x, -d.y);");
+            }
+            w_.close();
+        }
+        if (alias_in_) {
+            w_.line("__syncthreads();");
+        }
+        // phase 2: store them in LDS
+        for (std::size_t b = 0; b < B; ++b) {
+            std::string q = "q" + std::to_string(b);
+            std::string s1 = "s1_" + std::to_string(b), s2 = "s2_" + std::to_string(b);
+            w_.open("if (active && " + q + " < " + u(Q) + ")");
+            if (half_) {
+                w_.line(lds(q + " % " + u(H)) + " = " + s1 + ";");
+                w_.line("if (" + q + " != 0u && 2u * " + q + " != " + u(H) + ") " +
+                        lds(u(H) + " - " + q) + " = " + s2 + ";");
+            } else {
+                w_.line(lds(q) + " = " + s1 + ";");
+                w_.line("if (" + q + " != 0u) " + lds(u(c_.N) + " - " + q) + " = " + s2 + ";");
             }
             w_.close();
         }
@@ -903,9 +931,12 @@ class kernel_gen {
         std::size_t const Q = half_ ? H / 2 + 1 : (c_.N - 1) / 2 + 1;
         std::size_t const B = (Q + c_.Nt - 1) / c_.Nt;
         std::string const half = f_.lit(0.5);
+        // phase 1: read the complex FFT output and form the two spectrum values per q
         for (std::size_t b = 0; b < B; ++b) {
             std::string q = "q" + std::to_string(b);
+            std::string o1 = "o1_" + std::to_string(b), o2 = "o2_" + std::to_string(b);
             w_.line("const unsigned " + q + " = t + " + u(b * c_.Nt) + ";");
+            w_.line(T2() + " " + o1 + " = " + f_.zero2() + ", " + o2 + " = " + f_.zero2() + ";");
             w_.open("if (active && " + q + " < " + u(Q) + ")");
             if (half_) {
                 w_.line("const " + T2() + " y1 = " + lds(q + " % " + u(H)) + ";");
@@ -920,18 +951,16 @@ class kernel_gen {
                     w_.line("const " + T2() + " z0 = c_add(a, bt);");
                     w_.line("const " + T2() + " z1 = mk(a.x - bt.x, bt.y - a.y);");
                     w_.open("if (" + q + " == 0u)");
-                    w_.line(gstore(ooff(q), "z0"));
-                    w_.line(gstore(ooff(u(H)), "mk(z1.x, -z0.x)"));
+                    w_.line(o1 + " = z0;");
+                    w_.line(o2 + " = mk(z1.x, -z0.x);");
                     w_.close();
                     w_.open("else");
-                    w_.line(gstore(ooff(q), "c_scale(z0, " + two + ")"));
-                    w_.line("if (2u * " + q + " != " + u(H) + ") " +
-                            gstore(ooff(u(H) + " - " + q), "c_scale(z1, " + two + ")"));
+                    w_.line(o1 + " = c_scale(z0, " + two + ");");
+                    w_.line(o2 + " = c_scale(z1, " + two + ");");
                     w_.close();
                 } else {
-                    w_.line(gstore(ooff(q), "c_add(a, bt)"));
-                    w_.line("if (2u * " + q + " != " + u(H) + ") " +
-                            gstore(ooff(u(H) + " - " + q), "mk(a.x - bt.x, bt.y - a.y)"));
+                    w_.line(o1 + " = c_add(a, bt);");
+                    w_.line(o2 + " = mk(a.x - bt.x, bt.y - a.y);");
                 }
             } else {
                 w_.line("const " + T2() + " y1 = " + lds(q) + ";");
@@ -942,9 +971,26 @@ class kernel_gen {
                 std::string w = adjoint() ? "(" + q + " == 0u ? " + half + " : " +
                                                 f_.lit(1.0) + ")"
                                           : half;
-                w_.line(gstore(ooff(q), "c_scale(c_add(y2, y1), " + w + ")"));
-                w_.line("if (active_b) { const " + T2() + " d = c_scale(c_sub(y2, y1), " + w +
-                        "); " + gstore(ooff(q, 1), "mk(-d.y, d.x)") + " }");
+                w_.line(o1 + " = c_scale(c_add(y2, y1), " + w + ");");
+                w_.line("const " + T2() + " d = c_scale(c_sub(y2, y1), " + w + ");");
+                w_.line(o2 + " = mk(-d.y, d.x);");
+            }
+            w_.close();
+        }
+        if (alias_out_) {
+            w_.line("__syncthreads();");
+        }
+        // phase 2: store the half spectrum (q = 0 also writes the Nyquist / last bin H)
+        for (std::size_t b = 0; b < B; ++b) {
+            std::string q = "q" + std::to_string(b);
+            std::string o1 = "o1_" + std::to_string(b), o2 = "o2_" + std::to_string(b);
+            w_.open("if (active && " + q + " < " + u(Q) + ")");
+            if (half_) {
+                w_.line(gstore(ooff(q), o1));
+                w_.line("if (2u * " + q + " != " + u(H) + ") " + gstore(ooff(u(H) + " - " + q), o2));
+            } else {
+                w_.line(gstore(ooff(q), o1));
+                w_.line("if (active_b) " + gstore(ooff(q, 1), o2));
             }
             w_.close();
         }
@@ -956,6 +1002,7 @@ class kernel_gen {
     fp_helper f_;
     std::string name_;
     std::size_t group_size_ = 0;
+    bool alias_in_ = false, alias_out_ = false;
     std::string band_;
     writer w_;
     std::size_t nfft_ = 0, L_ = 0, MG_ = 1, np_ = 0;
@@ -1034,6 +1081,19 @@ void generate_fft1d_body(std::ostream &os, fft1d_configuration const &cfg,
         throw bad_configuration("ragged batches support c2c transforms only");
     }
     kernel_gen g(cfg, name, group_size);
+    os << g.generate_body();
+}
+
+void generate_fft1d_body_aliased(std::ostream &os, fft1d_configuration const &cfg,
+                                 std::string const &name, std::size_t group_size, bool alias_in,
+                                 bool alias_out) {
+    if (cfg.load_function || cfg.store_function) {
+        throw bad_configuration("callbacks are not supported in fused kernels");
+    }
+    if (cfg.flags & (fft1d_input_f16 | fft1d_input_bf16 | fft1d_mfma)) {
+        throw bad_configuration("half-precision input is not supported in fused kernels");
+    }
+    kernel_gen g(cfg, name, group_size, alias_in, alias_out);
     os << g.generate_body();
 }
 

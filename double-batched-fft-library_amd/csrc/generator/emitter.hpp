@@ -16,6 +16,11 @@ std::string hip_prelude();
 /// work-group of `group_size` work-items (>= cfg.work_group_size()).
 void generate_fft1d_body(std::ostream &os, fft1d_configuration const &cfg,
                          std::string const &name, std::size_t group_size);
+/// Tile body whose input (alias_in) or output (alias_out) pointer may point into the LDS
+/// scratch passed to the body (real transforms included).
+void generate_fft1d_body_aliased(std::ostream &os, fft1d_configuration const &cfg,
+                                 std::string const &name, std::size_t group_size, bool alias_in,
+                                 bool alias_out);
 
 } // namespace bbfft
 

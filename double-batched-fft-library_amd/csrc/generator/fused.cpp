@@ -1,0 +1,160 @@
+// Two consecutive passes of a multi-dimensional plan in one kernel (plane kept in LDS).
+//
+// Replaces, where the plane fits, two of the chained 1-D passes of the reference's nd_fft
+// (src/common/algorithm/nd_fft.hpp:52-99), which each make a full round trip to memory.
+#include "bbfft/bad_configuration.hpp"
+#include "bbfft/detail/generator_impl.hpp"
+#include "emitter.hpp"
+#include "heuristics.hpp"
+
+#include <algorithm>
+#include <iomanip>
+#include <ostream>
+#include <sstream>
+
+namespace bbfft {
+
+namespace {
+
+std::uint64_t fnv1a64(std::string const &s) {
+    std::uint64_t h = 1469598103934665603ull;
+    for (unsigned char ch : s) {
+        h ^= ch;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+/// Kernel configuration of pass `p` restricted to one plane (`kcols` K-columns, one tile),
+/// with the fewest work-items per transform that still fit `wg_max`; false if none fits.
+bool one_tile(configuration p, std::size_t kcols, device_info const &info, unsigned flags,
+              unsigned variant, std::size_t wg_max, fft1d_configuration &out) {
+    p.shape[2] = kcols;
+    std::size_t const M = p.shape[0];
+    bool const real = p.type != transform_type::c2c;
+    std::size_t const N = p.shape[1];
+    std::size_t const nfft = real && N % 2 == 0 ? N / 2 : N;
+    std::size_t const cols = real && N % 2 == 1 ? (kcols + 1) / 2 : kcols;
+    int const fp_max = p.fp == precision::f32 ? 32 : 16;
+    for (int rmax : {8, 16, 32, 64}) {
+        if (rmax > fp_max && rmax != 64) {
+            continue;
+        }
+        auto f = gen::default_factorization(nfft, static_cast<std::size_t>(rmax),
+                                            static_cast<std::size_t>(rmax));
+        std::size_t const r = static_cast<std::size_t>(*std::max_element(f.begin(), f.end()));
+        if (r > 64) {
+            return false;
+        }
+        std::size_t const Nt = nfft / r;
+        if (M * Nt * cols > wg_max) {
+            continue;
+        }
+        fft1d_tuning t;
+        t.factorization = f;
+        t.Nt = Nt;
+        t.Mb = M;
+        t.Kb = cols;
+        t.flags = flags;
+        t.has_flags = true;
+        try {
+            out = configure_fft1d(p, info, t, variant);
+        } catch (bad_configuration const &) {
+            return false;
+        }
+        return true;
+    }
+    return false;
+}
+
+} // namespace
+
+std::string fused2_configuration::identifier() const {
+    std::ostringstream oss;
+    oss << "bbfft_fused2_f" << static_cast<int>(a.fp) * 8 << "_G" << group_size << "_"
+        << std::hex << std::setw(16) << std::setfill('0')
+        << fnv1a64(a.identifier() + ";" + b.identifier() + ";" + std::to_string(ka) + ";" +
+                   std::to_string(kb) + ";" + std::to_string(plane_elems) + ";" +
+                   std::to_string(in_plane_stride) + ";" + std::to_string(out_plane_stride));
+    return oss.str();
+}
+
+std::size_t fused2_configuration::lds_bytes() const {
+    std::size_t const cb = 2 * static_cast<std::size_t>(a.fp);
+    return std::max({plane_elems * cb, a.lds_bytes(), b.lds_bytes()});
+}
+
+bool configure_fused2(configuration const &pa, configuration const &pb, device_info const &info,
+                      unsigned variant_a, unsigned variant_b, fused2_configuration &out) {
+    if (pa.dim != 1 || pb.dim != 1 || pa.fp != pb.fp || pa.callbacks || pb.callbacks) {
+        return false;
+    }
+    if (pa.type == transform_type::c2r || pb.type == transform_type::r2c) {
+        return false; // the intermediate must be complex
+    }
+    std::size_t const Ma = pa.shape[0], Na = pa.shape[1];
+    std::size_t const Mb = pb.shape[0], Nb = pb.shape[1];
+    std::size_t const na_out = pa.type == transform_type::r2c ? Na / 2 + 1 : Na;
+    std::size_t const nb_in = pb.type == transform_type::c2r ? Nb / 2 + 1 : Nb;
+    // packed intermediate: pass a writes {1, Ma, Ma na_out}, pass b reads {1, Mb, Mb nb_in}
+    if (pa.ostride[0] != 1 || pa.ostride[1] != Ma || pa.ostride[2] != Ma * na_out ||
+        pb.istride[0] != 1 || pb.istride[1] != Mb || pb.istride[2] != Mb * nb_in) {
+        return false;
+    }
+    std::size_t const P = std::max(pa.ostride[2], pb.istride[2]);
+    if (P % pa.ostride[2] || P % pb.istride[2]) {
+        return false;
+    }
+    std::size_t const ka = P / pa.ostride[2], kb = P / pb.istride[2];
+    if (pa.shape[2] % ka || pb.shape[2] % kb || pa.shape[2] / ka != pb.shape[2] / kb) {
+        return false;
+    }
+    std::size_t const wg_max = std::min<std::size_t>(1024, info.max_work_group_size);
+    fused2_configuration fc = {};
+    // streaming hints on the global side only
+    if (!one_tile(pa, ka, info, fft1d_nontemporal_load, variant_a, wg_max, fc.a) ||
+        !one_tile(pb, kb, info, fft1d_nontemporal_store, variant_b, wg_max, fc.b)) {
+        return false;
+    }
+    fc.ka = ka;
+    fc.kb = kb;
+    fc.planes = pa.shape[2] / ka;
+    fc.plane_elems = P;
+    fc.in_plane_stride = ka * pa.istride[2];
+    fc.out_plane_stride = kb * pb.ostride[2];
+    fc.group_size = std::max(fc.a.work_group_size(), fc.b.work_group_size());
+    if (fc.lds_bytes() > info.local_memory_size) {
+        return false;
+    }
+    out = fc;
+    return true;
+}
+
+void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::string_view name) {
+    std::string const n = name.empty() ? fc.identifier() : std::string(name);
+    bool const f32 = fc.a.fp == precision::f32;
+    std::string const T = f32 ? "float" : "double";
+    std::string const T2 = f32 ? "float2" : "double2";
+    std::string const in_t = fc.a.type == transform_type::r2c ? T : T2;
+    std::string const out_t = fc.b.type == transform_type::c2r ? T : T2;
+    std::size_t const lds_elems = fc.lds_bytes() / (2 * static_cast<std::size_t>(fc.a.fp));
+    // body names derive from the kernel name: several kernels may share one source
+    std::string const fa = n + "_a", fb = n + "_b";
+    os << hip_prelude();
+    generate_fft1d_body_aliased(os, fc.a, fa, fc.group_size, false, true);
+    generate_fft1d_body_aliased(os, fc.b, fb, fc.group_size, true, false);
+    os << "extern \"C\" __global__ void __launch_bounds__(" << fc.group_size << ") " << n
+       << "(const " << in_t << "* in, " << out_t << "* out, unsigned long planes) {\n"
+       << "  __shared__ " << T2 << " lds_buf[" << lds_elems << "];\n"
+       << "  for (unsigned long f = blockIdx.x; f < planes; f += gridDim.x) {\n"
+       << "    " << fa << "(in + f * " << fc.in_plane_stride << "ul, lds_buf, " << fc.ka
+       << "ul, 0ul, lds_buf, threadIdx.x);\n"
+       << "    __syncthreads();\n"
+       << "    " << fb << "(lds_buf, out + f * " << fc.out_plane_stride << "ul, " << fc.kb
+       << "ul, 0ul, lds_buf, threadIdx.x);\n"
+       << "    __syncthreads();\n"
+       << "  }\n"
+       << "}\n";
+}
+
+} // namespace bbfft

@@ -47,7 +47,9 @@ std::uint64_t fnv1a(std::string const &s) {
     return h;
 }
 
-std::string variant_name(std::size_t i) { return "bbfft_rv" + std::to_string(i); }
+std::string variant_name(std::string const &kernel, std::size_t i) {
+    return kernel + "_v" + std::to_string(i);
+}
 
 } // namespace
 
@@ -236,7 +238,7 @@ void generate_ragged(std::ostream &os, ragged_kernel_configuration const &rk,
     std::string const T2 = rk.fp == precision::f32 ? "float2" : "double2";
     os << hip_prelude();
     for (std::size_t v = 0; v < rk.variants.size(); ++v) {
-        generate_fft1d_body(os, rk.variants[v], variant_name(v), rk.group_size);
+        generate_fft1d_body(os, rk.variants[v], variant_name(n, v), rk.group_size);
     }
     std::size_t const lds_elems = std::max<std::size_t>(1, rk.lds_bytes() / (2 * static_cast<std::size_t>(rk.fp)));
     os << "extern \"C\" __global__ void __launch_bounds__(" << rk.group_size << ") " << n
@@ -265,7 +267,8 @@ void generate_ragged(std::ostream &os, ragged_kernel_configuration const &rk,
        << "    __asm__ volatile(\"\" : \"+v\"(lid));\n"
        << "    switch ((unsigned)rec[3]) {\n";
     for (std::size_t v = 0; v < rk.variants.size(); ++v) {
-        os << "    case " << v << "u: " << variant_name(v) << "(jin, jout, K, lt, lds_buf, lid); break;\n";
+        os << "    case " << v << "u: " << variant_name(n, v)
+           << "(jin, jout, K, lt, lds_buf, lid); break;\n";
     }
     os << "    default: break;\n"
        << "    }\n"

@@ -55,7 +55,24 @@ class fft1d_plan final : public plan_base {
     std::size_t in_align_ = 1, out_align_ = 1;
 };
 
-/// 2-D/3-D transform as a chain of 1-D passes on one stream.
+/// Two passes of a 2-D/3-D plan in one kernel (planes kept in LDS).
+class fused2_kernel {
+  public:
+    fused2_kernel(fused2_configuration const &fc, hipStream_t stream, int device,
+                  jit_cache *cache);
+    void launch(void const *in, void *out);
+    launch_info info() const;
+
+  private:
+    hipStream_t stream_;
+    fused2_configuration fc_;
+    std::string name_;
+    shared_handle<module_handle_t> mod_;
+    hipFunction_t fn_ = nullptr;
+    unsigned grid_ = 0;
+};
+
+/// 2-D/3-D transform as a chain of 1-D passes on one stream; two of them may run fused.
 class nd_plan final : public plan_base {
   public:
     nd_plan(pass_layout const &layout, hipStream_t stream, int device, jit_cache *cache,
@@ -67,7 +84,9 @@ class nd_plan final : public plan_base {
 
   private:
     hipStream_t stream_;
-    std::vector<std::unique_ptr<fft1d_plan>> passes_;
+    std::vector<std::unique_ptr<fft1d_plan>> passes_; ///< null where a fused kernel runs
+    std::unique_ptr<fused2_kernel> fused_;
+    std::size_t fused_first_ = static_cast<std::size_t>(-1);
     void *tmp_ = nullptr;
 };
 

@@ -106,10 +106,62 @@ std::vector<launch_info> fft1d_plan::launches() const {
     return {launch_info{name_, grid_, block_, kc_.lds_bytes()}};
 }
 
+fused2_kernel::fused2_kernel(fused2_configuration const &fc, hipStream_t stream, int device,
+                             jit_cache *cache)
+    : stream_(stream), fc_(fc) {
+    name_ = fc_.identifier();
+    jit_cache_key key{name_, hip::get_device_id(device)};
+    if (cache) {
+        mod_ = cache->get(key);
+    }
+    if (!mod_) {
+        std::ostringstream src;
+        generate_fused2(src, fc_);
+        hipModule_t m = hip::build_kernel_bundle(src.str(), device);
+        mod_ = shared_handle<module_handle_t>(reinterpret_cast<module_handle_t>(m),
+                                              &release_module);
+        if (cache) {
+            cache->store(key, mod_);
+        }
+    }
+    fn_ = hip::create_kernel(reinterpret_cast<hipModule_t>(mod_.get()), name_);
+    grid_ = static_cast<unsigned>(std::min<std::size_t>(fc_.planes, std::size_t(1) << 24));
+}
+
+void fused2_kernel::launch(void const *in, void *out) {
+    std::size_t const rb = static_cast<std::size_t>(fc_.a.fp);
+    std::size_t const ia = fc_.a.type == transform_type::r2c ? rb : 2 * rb;
+    std::size_t const oa = fc_.b.type == transform_type::c2r ? rb : 2 * rb;
+    if (reinterpret_cast<std::uintptr_t>(in) % ia || reinterpret_cast<std::uintptr_t>(out) % oa) {
+        throw bad_configuration("Input/output pointers are not aligned to the element size");
+    }
+    void const *a0 = in;
+    void *a1 = out;
+    unsigned long planes = fc_.planes;
+    void *args[] = {&a0, &a1, &planes};
+    BBFFT_HIP_CHECK(hipModuleLaunchKernel(fn_, grid_, 1, 1,
+                                          static_cast<unsigned>(fc_.group_size), 1, 1, 0,
+                                          stream_, args, nullptr));
+}
+
+launch_info fused2_kernel::info() const {
+    return launch_info{name_, grid_, static_cast<unsigned>(fc_.group_size), fc_.lds_bytes()};
+}
+
 nd_plan::nd_plan(pass_layout const &layout, hipStream_t stream, int device, jit_cache *cache,
                  unsigned real_pass_variant)
     : stream_(stream) {
-    for (auto const &p : layout.passes) {
+    fused_pair fp;
+    if (find_fused_pair(layout, hip::get_device_info(device), real_pass_variant, fp)) {
+        fused_first_ = fp.first;
+        fused_ = std::make_unique<fused2_kernel>(fp.fc, stream, device, cache);
+    }
+    for (std::size_t i = 0; i < layout.passes.size(); ++i) {
+        auto const &p = layout.passes[i];
+        if (fused_ && (i == fused_first_ || i == fused_first_ + 1)) {
+            passes_.push_back(nullptr);
+            continue;
+        }
         unsigned v = p.type == transform_type::c2c ? 0u : real_pass_variant;
         passes_.push_back(std::make_unique<fft1d_plan>(p, stream, device, cache,
                                                        fft1d_tuning{}, v));
@@ -130,14 +182,23 @@ void nd_plan::execute(void const *in, void *out, hipEvent_t signal, std::uint32_
     for (std::uint32_t i = 0; i < nwait; ++i) {
         BBFFT_HIP_CHECK(hipStreamWaitEvent(stream_, wait[i], 0));
     }
-    // All passes run on one stream, so stream order replaces the reference's internal events.
+    // All steps run on one stream, so stream order replaces the reference's internal events.
+    // Step 0 reads `in`, the last step writes `out`, the others work in the scratch buffer (or
+    // in `out` when the input is not larger than the output).
     void *tmp = tmp_ ? tmp_ : out;
     std::size_t const D = passes_.size();
-    passes_[0]->launch(in, tmp);
-    for (std::size_t d = 1; d + 1 < D; ++d) {
-        passes_[d]->launch(tmp, tmp);
+    std::size_t const steps = fused_ ? D - 1 : D;
+    std::size_t step = 0;
+    for (std::size_t i = 0; i < D; ++i, ++step) {
+        void const *src = step == 0 ? in : tmp;
+        void *dst = step + 1 == steps ? out : tmp;
+        if (fused_ && i == fused_first_) {
+            fused_->launch(src, dst);
+            ++i;
+        } else {
+            passes_[i]->launch(src, dst);
+        }
     }
-    passes_[D - 1]->launch(tmp, out);
     if (signal) {
         BBFFT_HIP_CHECK(hipEventRecord(signal, stream_));
     }
@@ -145,8 +206,13 @@ void nd_plan::execute(void const *in, void *out, hipEvent_t signal, std::uint32_
 
 std::vector<launch_info> nd_plan::launches() const {
     std::vector<launch_info> all;
-    for (auto const &p : passes_) {
-        auto l = p->launches();
+    for (std::size_t i = 0; i < passes_.size(); ++i) {
+        if (fused_ && i == fused_first_) {
+            all.push_back(fused_->info());
+            ++i;
+            continue;
+        }
+        auto l = passes_[i]->launches();
         all.insert(all.end(), l.begin(), l.end());
     }
     return all;

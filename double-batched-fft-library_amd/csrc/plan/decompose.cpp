@@ -2,6 +2,7 @@
 #include "bbfft/bad_configuration.hpp"
 #include "bbfft/detail/generator_impl.hpp"
 
+#include <cstdlib>
 #include <string>
 #include <utility>
 
@@ -101,6 +102,35 @@ pass_layout decompose(configuration const &cfg) {
         out.tmp_bytes = isize;
     }
     return out;
+}
+
+bool find_fused_pair(pass_layout const &layout, device_info const &info,
+                     unsigned real_pass_variant, fused_pair &out) {
+    char const *env = std::getenv("BBFFT_FUSE");
+    if (env && std::string(env) == "off") {
+        return false;
+    }
+    std::size_t const D = layout.passes.size();
+    if (D < 2) {
+        return false;
+    }
+    bool const c2r = layout.passes[D - 1].type == transform_type::c2r;
+    std::size_t const first = c2r ? D - 2 : 0;
+    auto const &pa = layout.passes[first];
+    auto const &pb = layout.passes[first + 1];
+    unsigned const va = pa.type == transform_type::c2c ? 0u : real_pass_variant;
+    unsigned const vb = pb.type == transform_type::c2c ? 0u : real_pass_variant;
+    fused2_configuration fc;
+    if (!configure_fused2(pa, pb, info, va, vb, fc)) {
+        return false;
+    }
+    // One work-group per plane: with few large planes the unfused passes spread the work over
+    // more work-groups; small problems are launch-bound and gain from one launch less.
+    if (fc.planes < 256 && fc.planes * fc.plane_elems > (std::size_t(1) << 20)) {
+        return false;
+    }
+    out = fused_pair{first, fc};
+    return true;
 }
 
 configuration adjoint_configuration(configuration const &cfg, unsigned *real_pass_variant) {

@@ -3,6 +3,8 @@
 #define BBFFT_AMD_DECOMPOSE_HPP
 
 #include "bbfft/configuration.hpp"
+#include "bbfft/detail/generator_impl.hpp"
+#include "bbfft/device_info.hpp"
 
 #include <cstddef>
 #include <vector>
@@ -33,6 +35,20 @@ pass_layout decompose(configuration const &cfg);
  * (Hermitian-fold weighting), which `real_pass_variant` reports.
  */
 configuration adjoint_configuration(configuration const &cfg, unsigned *real_pass_variant);
+
+/// Two consecutive passes of a layout that run as one fused kernel.
+struct fused_pair {
+    std::size_t first; ///< index of the first fused pass
+    fused2_configuration fc;
+};
+
+/**
+ * @brief The pair of passes of a 2-D/3-D layout to fuse on this device, if any: the first two
+ * passes (modes 1 and 2) for c2c and r2c, the last two for c2r (the passes run in reverse
+ * mode order). `real_pass_variant` applies to the real pass. BBFFT_FUSE=off disables fusion.
+ */
+bool find_fused_pair(pass_layout const &layout, device_info const &info,
+                     unsigned real_pass_variant, fused_pair &out);
 
 } // namespace bbfft
 

@@ -116,6 +116,43 @@ BBFFT_EXPORT void generate_fft1d(std::ostream &os, fft1d_configuration const &cf
                                  user_module const *callbacks = nullptr);
 
 /**
+ * @brief Two consecutive passes of a 2-D/3-D plan fused into one kernel over "planes".
+ *
+ * The intermediate tensor between pass a and pass b is packed, so it splits into contiguous
+ * planes of `plane_elems` complex elements, each holding ka K-columns of pass a's output and kb
+ * K-columns of pass b's input. Work-group f computes pass a for its plane (one tile: all
+ * M-columns and all ka K-columns) into LDS, then pass b from LDS to global memory; the plane
+ * shares the LDS with the passes' stage scratch (barriers separate the uses). HBM traffic is
+ * one read of the input and one write of the output instead of two round trips.
+ */
+struct BBFFT_EXPORT fused2_configuration {
+    fft1d_configuration a;          ///< first pass (reads global memory, writes the plane)
+    fft1d_configuration b;          ///< second pass (reads the plane, writes global memory)
+    std::size_t ka;                 ///< K-columns of pass a per plane
+    std::size_t kb;                 ///< K-columns of pass b per plane
+    std::size_t planes;             ///< number of planes
+    std::size_t plane_elems;        ///< complex elements per plane
+    std::size_t in_plane_stride;    ///< input elements between planes (ka * a.istride[2])
+    std::size_t out_plane_stride;   ///< output elements between planes (kb * b.ostride[2])
+    std::size_t group_size;         ///< work-items per work-group
+
+    std::string identifier() const;
+    std::size_t lds_bytes() const;
+};
+
+/**
+ * @brief Configure the fused kernel for 1-D passes pa then pb of one plan (as decompose makes
+ * them); returns false when they cannot be fused on this device (plane or stage scratch over
+ * the LDS size, or a pass that does not fit one work-group).
+ */
+BBFFT_EXPORT bool configure_fused2(configuration const &pa, configuration const &pb,
+                                   device_info const &info, unsigned variant_a,
+                                   unsigned variant_b, fused2_configuration &out);
+/// Emit the HIP source of the fused kernel `name(in, out, planes)`.
+BBFFT_EXPORT void generate_fused2(std::ostream &os, fused2_configuration const &fc,
+                                  std::string_view name = {});
+
+/**
  * @brief Persistent kernel of a ragged batch.
  *
  * Every distinct (N, M, direction) of the jobs is a variant: an fft1d_configuration whose

@@ -180,6 +180,27 @@ BBFFT_EXPORT_C void bbfft_plan_destroy(bbfft_plan_t plan);
 BBFFT_EXPORT_C int bbfft_device_info_query(int device, char *buf, size_t len);
 
 /*
+ * Fused passes of 2-D/3-D plans (extension of the reference's nd_fft,
+ * src/common/algorithm/nd_fft.hpp:52-99, which runs one memory round trip per mode): where a
+ * plane of the intermediate tensor fits in LDS, the passes over modes 1 and 2 run as one
+ * kernel. These entry points report the decision and the kernel for a device_info (host only);
+ * plans apply it automatically (BBFFT_FUSE=off in the environment disables it).
+ */
+typedef struct bbfft_fused_info {
+    unsigned first_pass;
+    size_t group_size, lds_bytes, planes, ka, kb, plane_elems, in_plane_stride,
+        out_plane_stride;
+} bbfft_fused_info;
+
+BBFFT_EXPORT_C int bbfft_fused_configuration(const bbfft_configuration *cfg,
+                                             const char *device_info, int *fused,
+                                             bbfft_fused_info *info, char *name,
+                                             size_t name_len);
+BBFFT_EXPORT_C int bbfft_generate_fused_source(const bbfft_configuration *cfg,
+                                               const char *device_info, char *buf, size_t len,
+                                               size_t *needed);
+
+/*
  * Half-precision input (extension; no reference counterpart). For a 1-D c2c fp32
  * configuration, the input holds (re, im) pairs of IEEE binary16 (BBFFT_INPUT_F16) or bfloat16
  * (BBFFT_INPUT_BF16) with the configuration's strides (counted in complex elements, 4 bytes

@@ -101,6 +101,26 @@ def run_half_input(bbfft, cfg, x_bits, flags, grid_cap=None):
     return out
 
 
+def run_fused(bbfft, cfg, x, out, grid_cap=None):
+    """Execute the fused kernel of a 2-D/3-D configuration from host array x into host array
+    out (both in the layouts that kernel reads and writes)."""
+    info = bbfft.GFX950_INFO
+    fc = bbfft.fused_configuration(cfg, info)
+    src = bbfft.generate_fused_source(cfg, info)
+    f32 = int(cfg.fp) == 4
+    real_t, cplx_t = ("float", "float2") if f32 else ("double", "double2")
+    first = fc["first_pass"]
+    t = int(cfg.type)
+    in_t = real_t if (t == 1 and first == 0) else cplx_t
+    out_t = real_t if t == 2 else cplx_t
+    fn = compile_kernel(src, fc["name"], in_t, out_t)
+    grid = fc["planes"] if grid_cap is None else min(fc["planes"], grid_cap)
+    rc = fn(x.ctypes.data, out.ctypes.data, fc["planes"], grid, fc["group_size"])
+    if rc != 0:
+        raise RuntimeError("divergent barrier in emulated kernel")
+    return out
+
+
 def run_ragged(ragged, jobs, x, out_size, precision=4, grid=7, in_place=False):
     """Execute the persistent ragged kernel for `jobs` on host arrays (`grid` work-groups)."""
     kc = ragged.kernel_configuration(jobs, precision)

@@ -1,0 +1,96 @@
+"""Fused passes of 2-D/3-D plans on the CPU (fiber emulator): the plane kernel (passes over
+modes 1 and 2 with the plane in LDS, sharing it with the stage scratch) chained with the
+remaining pass exactly as the plan chains them, against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import bbfft_amd as bb
+import oracle
+from emu import runner
+
+
+def _cfg(M, dims, K, fp, t, inplace=False):
+    d = 1 if t == 2 else -1
+    cfg = bb.configuration(len(dims), [M] + list(dims) + [K], bb.precision(fp),
+                           bb.direction(d), bb.transform_type(t))
+    cfg.set_strides_default(inplace)
+    return cfg
+
+
+def _run_plan(cfg, x, n_out):
+    """The plan's steps: the fused pair replaces two passes; other passes as usual."""
+    fc = bb.fused_configuration(cfg)
+    assert fc is not None
+    D = cfg.dim
+    fp = int(cfg.fp)
+    t = int(cfg.type)
+    ct = np.complex64 if fp == 4 else np.complex128
+    rt = np.float32 if fp == 4 else np.float64
+    M, K = cfg.shape[0], cfg.shape[D + 1]
+    n_c = [cfg.shape[1] // 2 + 1 if t != 0 else cfg.shape[1]] + list(cfg.shape[2:D + 1])
+    inter = M * int(np.prod(n_c)) * K  # packed complex intermediate
+    first = fc["first_pass"]
+    if D == 2:
+        out = np.zeros(n_out, rt if t == 2 else ct)
+        return runner.run_fused(bb, cfg, x, out, grid_cap=3)
+    if first == 0:  # fused modes 1, 2 then the pass over mode 3
+        mid = np.zeros(inter, ct)
+        runner.run_fused(bb, cfg, np.ascontiguousarray(x), mid, grid_cap=5)
+        return runner.run(bb, cfg, mid, n_out, pass_index=2)
+    # c2r: the pass over mode 3 first, then modes 2, 1 fused
+    mid = runner.run(bb, cfg, x, inter, pass_index=0)
+    out = np.zeros(n_out, rt)
+    return runner.run_fused(bb, cfg, mid, out, grid_cap=5)
+
+
+@pytest.mark.parametrize("fp", [4, 8])
+@pytest.mark.parametrize("t", [0, 1, 2])
+@pytest.mark.parametrize("M,dims,K", [(1, [16, 12], 3), (2, [8, 6, 5], 2), (1, [10, 8, 4], 3),
+                                      (3, [6, 9], 2), (1, [15, 7, 4], 2)])
+def test_fused_vs_oracle(fp, t, M, dims, K):
+    cfg = _cfg(M, dims, K, fp, t)
+    shape = cfg.shape
+    D = len(dims)
+    half = list(shape)
+    half[1] = dims[0] // 2 + 1
+    ishape, oshape = (shape, half) if t == 1 else (half, shape) if t == 2 else (shape, shape)
+    n_in = oracle.extent(D, ishape, cfg.istride)
+    n_out = oracle.extent(D, oshape, cfg.ostride)
+    x = oracle.uniform_real(5, n_in) if t == 1 else oracle.uniform_complex(5, n_in)
+    if fp == 4:
+        x = x.astype(np.float32 if t == 1 else np.complex64)
+    ref = oracle.fft(D, shape, cfg.istride, cfg.ostride, t, int(cfg.dir), x, out_size=n_out)
+    y = _run_plan(cfg, x, n_out)
+    assert oracle.rel_l2(y, ref) <= oracle.tolerance(fp, int(np.prod(dims)))
+
+
+def test_fused_in_place_layout_r2c():
+    """In-place r2c layout (padded real rows) read by the fused kernel."""
+    cfg = _cfg(1, [16, 8, 4], 2, 4, 1, inplace=True)
+    n_in = oracle.extent(3, cfg.shape, cfg.istride)
+    half = list(cfg.shape)
+    half[1] = 9
+    n_out = oracle.extent(3, half, cfg.ostride)
+    x = oracle.uniform_real(6, n_in).astype(np.float32)
+    ref = oracle.fft(3, cfg.shape, cfg.istride, cfg.ostride, 1, -1, x, out_size=n_out)
+    y = _run_plan(cfg, x, n_out)
+    m, n1, n2, n3, k = np.meshgrid(range(1), range(9), range(8), range(4), range(2), indexing="ij")
+    sel = (n1 * cfg.ostride[1] + n2 * cfg.ostride[2] + n3 * cfg.ostride[3] +
+           k * cfg.ostride[4]).ravel()
+    assert oracle.rel_l2(y[sel], ref[sel]) <= oracle.tolerance(4, 512)
+
+
+def test_fusion_policy():
+    """The 3-D 32 x 128^3 real transforms fuse modes 1 and 2 (one 128 x 65 plane per work-group,
+    plane and stage scratch sharing 70 KB of LDS); 2-D planes that do not fit are not fused;
+    few large planes are left to the unfused passes."""
+    fc = bb.fused_configuration(bb.parse_fft_descriptor("srfo128x128x128*32"))
+    assert fc["first_pass"] == 0 and fc["planes"] == 128 * 32 and fc["plane_elems"] == 65 * 128
+    assert fc["lds_bytes"] <= 80 * 1024 and fc["group_size"] <= 1024
+    c2r = _cfg(1, [128, 128, 128], 32, 4, 2)
+    fr = bb.fused_configuration(c2r)
+    assert fr["first_pass"] == 1 and fr["kb"] == 128 and fr["ka"] == 1
+    assert bb.fused_configuration(bb.parse_fft_descriptor("scfo256x256*4")) is None
+    assert bb.fused_configuration(bb.parse_fft_descriptor("scfo128x128*300")) is not None
+    assert bb.fused_configuration(bb.parse_fft_descriptor("scfo128x128*100")) is None
+    assert bb.fused_configuration(bb.parse_fft_descriptor("scfo64x64*100")) is not None
