@@ -198,6 +198,42 @@ def mfma_split(bb, torch, cache, steps):
     return out
 
 
+def nd_real_3d(bb, torch, cache, steps):
+    """3-D real transforms of 32 x 128^3 fp32 (r2c forward and c2r backward, out of place):
+    the plan with modes 1 and 2 fused (one kernel per 128 x 65 plane) beside the unfused chain
+    of three passes (BBFFT_FUSE=off). Flops 2.5 N log2 N per transform, N = 128^3."""
+    n, B = 128, 32
+    N = n ** 3
+    flops = 2.5 * N * math.log2(N) * B
+    nbytes = B * (N * 4.0 + n * n * (n // 2 + 1) * 8.0)
+    x = torch.rand(B, n, n, n, device="cuda") * 2 - 1
+    X = torch.empty(B, n, n, n // 2 + 1, dtype=torch.complex64, device="cuda")
+    out = {}
+    for kind, desc, a, b in (("r2c", f"srfo{n}x{n}x{n}*{B}", x, X),
+                             ("c2r", f"srbo{n}x{n}x{n}*{B}", X, x)):
+        for fused in (True, False):
+            os.environ["BBFFT_FUSE"] = "on" if fused else "off"
+            plan = bb.make_plan(desc, cache=cache)
+            plan.execute(a, b)
+            torch.cuda.synchronize()
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record()
+            for _ in range(steps):
+                plan.execute(a, b)
+            e.record()
+            torch.cuda.synchronize()
+            ms = s.elapsed_time(e) / steps
+            out[f"{kind}_{'fused' if fused else 'unfused'}"] = {
+                "ms": round(ms, 4), "GFLOPs": round(flops / ms / 1e6, 1),
+                "GBps_algorithmic": round(nbytes / ms / 1e6, 1),
+                "frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBPS, 4),
+                "kernels": [l["kernel"] for l in plan.launches()]}
+    os.environ.pop("BBFFT_FUSE", None)
+    del x, X
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(budget_s=15.0):
     """Oracle port of the reference algorithm on host cores, bounded sample of the workload."""
     try:
@@ -216,6 +252,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ragged", action="store_true")
     ap.add_argument("--no-mfma", action="store_true")
+    ap.add_argument("--no-nd", action="store_true")
     ap.add_argument("--desc", default="scfo8.256*32768")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (counter calibration)")
@@ -244,6 +281,10 @@ def main():
                               "GFLOPs": round(r["gflops"], 1), "GBps": round(r["GBps"], 1),
                               "frac": round(r["frac"], 4)})
                 print(json.dumps(sweep[-1]), file=sys.stderr, flush=True)
+    nd = None
+    if not a.no_nd:
+        nd = nd_real_3d(bb, torch, cache, a.steps)
+        print(json.dumps({"nd_real_3d": nd}), file=sys.stderr, flush=True)
     mf = None
     if not a.no_mfma:
         mf = mfma_split(bb, torch, cache, a.steps)
@@ -272,7 +313,8 @@ def main():
                                f"x K={cfg.shape[2]}, out-of-place, uniform[-1,1) complex",
                    "descriptor": a.desc, "HBM_GBps": round(head["GBps"], 1),
                    "hbm_copy_GBps": round(copy_gbps, 1), "kernel": head["kernel"]["kernel"],
-                   "sweep": sweep, "mfma_split_n512": mf, "ragged_serving": rag},
+                   "sweep": sweep, "nd_real_3d_32x128cubed": nd, "mfma_split_n512": mf,
+                   "ragged_serving": rag},
         "roofline": {"bound": "hbm", "achieved": round(head["GBps"], 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(head["frac"], 4),
                      "traffic": MEASURED_TRAFFIC_BYTES.get(head["kernel"]["kernel"])},

@@ -32,27 +32,32 @@ def _divisors(n):
     return [d for d in range(1, n + 1) if n % d == 0]
 
 
-def candidate_factorizations(nfft, fp, limit=8):
-    """Ordered factorisations into 1-3 radices in [2, 32], fewest stages / most balanced first."""
+def candidate_factorizations(nfft, fp, per_length=4):
+    """Factorisations into 1-4 radices in [2, 32]: the `per_length` most balanced ones for each
+    number of stages (largest radix first), fewest stages first."""
     rmax = 32
-    cands = set()
-    if 2 <= nfft <= rmax:
-        cands.add((nfft,))
-    for a in _divisors(nfft):
-        b = nfft // a
-        if 2 <= a <= rmax and 2 <= b <= rmax:
-            cands.add((a, b))
-        for c in _divisors(b):
-            d = b // c
-            if 2 <= a <= rmax and 2 <= c <= rmax and 2 <= d <= rmax:
-                cands.add((a, c, d))
+
+    def splits(n, parts):
+        if parts == 1:
+            return [(n,)] if 2 <= n <= rmax else []
+        out = []
+        for a in _divisors(n):
+            if 2 <= a <= rmax:
+                out += [(a,) + rest for rest in splits(n // a, parts - 1)]
+        return out
+
     if nfft == 1:
-        cands.add((1,))
-
-    def score(f):
-        return (len(f), max(f) - min(f), -f[0])
-
-    return [list(f) for f in sorted(cands, key=score)[:limit]]
+        return [[1]]
+    result = []
+    for parts in range(1, 5):
+        multisets = {tuple(sorted(f, reverse=True)) for f in splits(nfft, parts)}
+        ranked = sorted(multisets, key=lambda f: (max(f) - min(f), -f[0]))[:per_length]
+        for m in ranked:
+            # stage orders: all of them up to 3 stages, largest-first and smallest-first for 4
+            orders = sorted(set(itertools.permutations(m)), reverse=True) if parts <= 3 \
+                else sorted({m, tuple(reversed(m))}, reverse=True)
+            result += [list(o) for o in orders]
+    return result
 
 
 def _pow2_upto(n):
@@ -215,11 +220,31 @@ def write_table(path, lines):
         f.write('R"TABLE(\n' + TABLE_HEADER + "\n".join(lines) + '\n)TABLE"\n')
 
 
+def _key(line):
+    """(precision, type, N, M) of a table line, or None for comments / blank lines."""
+    parts = line.split("#", 1)[0].split()
+    return tuple(parts[:4]) if len(parts) >= 5 and parts[4] == ":" else None
+
+
+def merge_table(path, new_lines):
+    """Entries of the table at `path` with those of `new_lines` replacing equal keys."""
+    old = []
+    if os.path.exists(path):
+        with open(path) as f:
+            text = f.read()
+        body = text.split('R"TABLE(', 1)[-1].split(')TABLE"', 1)[0]
+        old = [ln for ln in body.splitlines() if _key(ln)]
+    keys = {_key(ln) for ln in new_lines}
+    return [ln for ln in old if _key(ln) not in keys] + list(new_lines)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("-o", "--output", default=None, help="table file to write (.inc)")
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--merge", action="store_true",
+                    help="keep the other entries of an existing output table")
     ap.add_argument("descriptors", nargs="+")
     a = ap.parse_args(argv)
     lines = []
@@ -234,7 +259,7 @@ def main(argv=None):
                           "speedup": round(tuner.default_ms / best_ms, 4),
                           "seconds": round(time.time() - t0, 1), "line": line}), flush=True)
         if a.output:
-            write_table(a.output, lines)
+            write_table(a.output, merge_table(a.output, lines) if a.merge else lines)
         del tuner
     return 0
 

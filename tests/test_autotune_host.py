@@ -1,0 +1,65 @@
+"""Host logic of the autotuner: candidate factorisations, table lines and table merging, and
+that the compiled-in table parses into valid kernel configurations."""
+import math
+import os
+import re
+
+import pytest
+
+import bbfft_amd as bb
+from bbfft_amd import autotune
+
+TABLE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                     "double-batched-fft-library_amd", "tuning", "gfx950.inc")
+
+
+@pytest.mark.parametrize("n", [16, 60, 64, 105, 128, 243, 256, 384, 512])
+def test_candidates_multiply_to_n(n):
+    c = autotune.candidate_factorizations(n, 4)
+    assert c and all(math.prod(f) == n for f in c)
+    assert all(2 <= r <= 32 for f in c for r in f)
+    assert len({tuple(f) for f in c}) == len(c)
+    assert all(1 <= len(f) <= 4 for f in c)
+    assert [len(f) for f in c] == sorted(len(f) for f in c)  # fewest stages first
+
+
+def test_candidates_include_tuned_orders():
+    """Stage orders matter: the table's measured winners are among the candidates."""
+    assert [8, 8, 4] in autotune.candidate_factorizations(256, 4)
+    assert [16, 4, 8] in autotune.candidate_factorizations(512, 4)
+    assert [4, 4, 4, 4] in autotune.candidate_factorizations(256, 4)
+
+
+def _table_lines():
+    with open(TABLE) as f:
+        body = f.read().split('R"TABLE(', 1)[1].split(')TABLE"', 1)[0]
+    return [ln for ln in body.splitlines() if autotune._key(ln)]
+
+
+def test_table_entries_are_valid_configurations():
+    for ln in _table_lines():
+        prec, typ, N, M = autotune._key(ln)
+        fields = ln.split("#")[0].split(":")[1].split()
+        fac = [int(v) for v in fields[0].split("x")]
+        Nt, Mb, Kb, pad, flags = (int(v) for v in fields[1:6])
+        nfft = int(N) // 2 if typ != "c2c" and int(N) % 2 == 0 else int(N)
+        assert math.prod(fac) == nfft, ln
+        desc = f"{'s' if prec == 'f32' else 'd'}{'c' if typ == 'c2c' else 'r'}" \
+               f"{'b' if typ == 'c2r' else 'f'}o{M}.{N}*1000"
+        kc = bb.kernel_configuration(bb.parse_fft_descriptor(desc))
+        assert kc["factorization"] == fac and kc["Nt"] == Nt, ln
+        assert kc["Mb"] == min(Mb, int(M)) and kc["lds_pad"] == pad, ln
+
+
+def test_merge_and_line_format(tmp_path):
+    lines = _table_lines()
+    p = tmp_path / "t.inc"
+    autotune.write_table(str(p), lines)
+    new = "f32 c2c 256 8 : 4x4x4x4 64 8 1 16 3   # 1 -> 2 GB/s, scfo8.256*32768"
+    merged = autotune.merge_table(str(p), [new])
+    assert len(merged) == len(lines) and merged[-1] == new
+    assert sum(autotune._key(ln) == ("f32", "c2c", "256", "8") for ln in merged) == 1
+    cfg = bb.parse_fft_descriptor("scfo8.256*32768")
+    t = {"factorization": [8, 8, 4], "Nt": 32, "Mb": 8, "Kb": 1, "lds_pad": 16, "flags": 3}
+    line = autotune.table_line(cfg, t, 1.0, 0.5, 1e9, "scfo8.256*32768")
+    assert re.match(r"f32 c2c 256 8 : 8x8x4 32 8 1 16 3 +# 1000 -> 2000 GB/s", line)
