@@ -282,4 +282,14 @@ auto make_plan(configuration const &cfg, hipStream_t stream, jit_cache *cache) -
     return hip_plan(hip_detail::create(cfg, stream, cache));
 }
 
+auto make_half_input_plan(configuration const &cfg, unsigned flags, hipStream_t stream,
+                          jit_cache *cache) -> hip_plan {
+    unsigned const allowed = fft1d_input_f16 | fft1d_input_bf16 | fft1d_mfma;
+    if ((flags & ~allowed) || !(flags & (fft1d_input_f16 | fft1d_input_bf16))) {
+        throw bad_configuration("Half-input plans take fft1d_input_f16 or fft1d_input_bf16, "
+                                "optionally with fft1d_mfma");
+    }
+    return hip_plan(hip_detail::create_variant(cfg, flags, stream, cache));
+}
+
 } // namespace bbfft

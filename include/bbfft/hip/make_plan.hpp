@@ -27,6 +27,15 @@ BBFFT_EXPORT auto make_plan(configuration const &cfg, hipStream_t stream,
                             jit_cache *cache = nullptr) -> hip_plan;
 
 /**
+ * @brief Plan of a 1-D c2c fp32 configuration whose input is stored in half precision
+ * (extension): `flags` is fft1d_input_f16 or fft1d_input_bf16, optionally with fft1d_mfma
+ * (radix-16 first stage on the matrix cores). The output is fp32 complex.
+ */
+BBFFT_EXPORT auto make_half_input_plan(configuration const &cfg, unsigned flags,
+                                       hipStream_t stream, jit_cache *cache = nullptr)
+    -> hip_plan;
+
+/**
  * @brief Plan for a ragged batch (extension): all jobs in one persistent kernel launch.
  *
  * execute(in, out) runs every job on its slice of in / out. With use_graph the launch is
