@@ -68,8 +68,12 @@ def _pow2_upto(n):
     return out
 
 
+STORAGE_BITS = 8 | 16 | 32  # half-precision input (f16 / bf16) and the MFMA stage
+
+
 class Tuner:
-    def __init__(self, desc, reps=10, verbose=True):
+    def __init__(self, desc, reps=10, verbose=True, variant=0):
+        """`variant`: half-input kernel bits (8 f16, 16 bf16, | 32 MFMA stage) or 0."""
         import torch
         self.torch = torch
         self.desc = desc
@@ -78,6 +82,7 @@ class Tuner:
             raise ValueError("autotuning applies to 1-D descriptors")
         self.reps = reps
         self.verbose = verbose
+        self.variant = variant & STORAGE_BITS
         self.cache = default_cache()
         M, N, K = self.cfg.shape[:3]
         self.M, self.N, self.K = M, N, K
@@ -96,10 +101,18 @@ class Tuner:
         n_out = sum((oshape[i] - 1) * ostride[i] for i in range(3)) + 1
         in_elems = n_in if t == 1 else 2 * n_in
         out_elems = n_out if t == 2 else 2 * n_out
-        self.x = torch.rand(in_elems, generator=g, device="cuda", dtype=rdt) * 2 - 1
+        if self.variant:
+            # (re, im) half pairs as int32 words; uniform[-1, 1) values
+            hdt = torch.float16 if self.variant & 8 else torch.bfloat16
+            v = (torch.rand(2 * n_in, generator=g, device="cuda") * 2 - 1).to(hdt)
+            self.x = v.view(torch.int32)
+            in_bytes = 4 * n_in
+        else:
+            self.x = torch.rand(in_elems, generator=g, device="cuda", dtype=rdt) * 2 - 1
+            in_bytes = in_elems * (4 if self.fp == 4 else 8)
         self.y = torch.empty(out_elems, device="cuda", dtype=rdt)
         self.ref = torch.empty_like(self.y)
-        self.bytes = (in_elems + out_elems) * (4 if self.fp == 4 else 8)
+        self.bytes = in_bytes + out_elems * (4 if self.fp == 4 else 8)
         self.default_ms = None
         self.results = {}
 
@@ -151,16 +164,24 @@ class Tuner:
         return ms
 
     def full_tuning(self, partial):
-        """Complete a partial tuning with the values the library would choose for it."""
+        """Complete a partial tuning with the values the library would choose for it (the
+        variant bits are part of every tuning's flags)."""
+        partial = dict(partial, flags=partial.get("flags", 0) | self.variant)
         kc = kernel_configuration(self.cfg, tuning=partial, device_info=GFX950_INFO)
         return {"factorization": kc["factorization"], "Nt": kc["Nt"], "Mb": kc["Mb"],
                 "Kb": kc["Kb"], "lds_pad": kc["lds_pad"], "no_pad": kc["lds_pad"] == 0,
-                "flags": partial.get("flags", 0)}
+                "flags": partial["flags"]}
+
+    def factorizations(self):
+        if self.variant & 32:  # the MFMA stage is a first radix of 16
+            return [[16] + f for f in candidate_factorizations(self.nfft // 16, self.fp)
+                    if f != [1]]
+        return candidate_factorizations(self.nfft, self.fp)
 
     def run(self, rounds=2):
         torch = self.torch
         # reference output and time of the heuristic default (explicit, table-independent)
-        base = self.full_tuning({"flags": 0})
+        base = self.full_tuning({"flags": self.variant})
         plan = make_plan(self.cfg, cache=self.cache, tuning=base)
         plan.execute(self.x, self.ref)
         torch.cuda.synchronize()
@@ -173,8 +194,8 @@ class Tuner:
         for _ in range(rounds):
             # 1) factorisation and work-items per transform (tile re-derived by the library)
             opts = []
-            for f in candidate_factorizations(self.nfft, self.fp):
-                nt0 = self.nfft // max(f)
+            for f in self.factorizations():
+                nt0 = self.nfft // max(f[1:] if self.variant & 32 else f)
                 for nt in sorted({nt0, 2 * nt0}):
                     if nt <= 256:
                         opts.append({"factorization": f, "Nt": nt, "flags": best["flags"],
@@ -192,14 +213,14 @@ class Tuner:
             opts = [dict(best, lds_pad=p, no_pad=p == 0) for p in (0, 8, 16, 32)]
             best, best_ms = self._try(best, best_ms, opts)
             # 5) streaming hints
-            opts = [dict(best, flags=fl) for fl in (0, 1, 2, 3)]
+            opts = [dict(best, flags=fl | self.variant) for fl in (0, 1, 2, 3)]
             best, best_ms = self._try(best, best_ms, opts)
         return best, best_ms
 
     def _try(self, best, best_ms, options, complete=False):
         for o in options:
             t = self.full_tuning(o) if complete else o
-            t["flags"] = o.get("flags", 0)
+            t["flags"] = o.get("flags", 0) | self.variant
             ms = self.evaluate(t)
             if ms < best_ms * 0.995:  # keep the incumbent unless clearly faster
                 best, best_ms = t, ms
@@ -209,9 +230,16 @@ class Tuner:
 def table_line(cfg, tuning, default_ms, best_ms, nbytes, desc):
     prec = "f32" if int(cfg.fp) == 4 else "f64"
     typ = ["c2c", "r2c", "c2r"][int(cfg.type)]
+    flags = tuning["flags"]
+    if flags & 8:
+        typ += "-f16"
+    if flags & 16:
+        typ += "-bf16"
+    if flags & 32:
+        typ += "-mfma"
     fac = "x".join(str(f) for f in tuning["factorization"])
     return (f"{prec} {typ} {cfg.shape[1]} {cfg.shape[0]} : {fac} {tuning['Nt']} {tuning['Mb']} "
-            f"{tuning['Kb']} {tuning['lds_pad']} {tuning['flags']}   # "
+            f"{tuning['Kb']} {tuning['lds_pad']} {flags & 3}   # "
             f"{nbytes / default_ms / 1e6:.0f} -> {nbytes / best_ms / 1e6:.0f} GB/s, {desc}")
 
 
@@ -245,12 +273,18 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--merge", action="store_true",
                     help="keep the other entries of an existing output table")
+    ap.add_argument("--storage", choices=["fp32", "f16", "bf16"], default="fp32",
+                    help="input storage of c2c fp32 descriptors (half-input kernels)")
+    ap.add_argument("--mfma", action="store_true", help="with --storage f16/bf16: MFMA stage")
     ap.add_argument("descriptors", nargs="+")
     a = ap.parse_args(argv)
     lines = []
     for desc in a.descriptors:
         t0 = time.time()
-        tuner = Tuner(desc, reps=a.reps)
+        variant = {"fp32": 0, "f16": 8, "bf16": 16}[a.storage]
+        if variant and a.mfma:
+            variant |= 32
+        tuner = Tuner(desc, reps=a.reps, variant=variant)
         best, best_ms = tuner.run(a.rounds)
         line = table_line(tuner.cfg, best, tuner.default_ms, best_ms, tuner.bytes, desc)
         lines.append(line)

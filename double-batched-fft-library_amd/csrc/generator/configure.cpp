@@ -120,10 +120,8 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     unsigned const all_flags = (tuning_in.has_flags ? tuning_in.flags : 0u) | variant;
     bool const half_in = (all_flags & (fft1d_input_f16 | fft1d_input_bf16)) != 0;
     bool const mfma = (all_flags & fft1d_mfma) != 0;
-    // The table holds kernels measured with fp32 storage.
-    if (!explicit_tuning && !half_in && info.type == device_type::gpu &&
-        info.max_subgroup_size() == 64) {
-        gen::lookup_tuning(cfg.fp, cfg.type, N, M, tuning);
+    if (!explicit_tuning && info.type == device_type::gpu && info.max_subgroup_size() == 64) {
+        gen::lookup_tuning(cfg.fp, cfg.type, N, M, all_flags, tuning);
     }
 
     fft1d_configuration c = {};

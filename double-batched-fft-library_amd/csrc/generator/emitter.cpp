@@ -642,47 +642,67 @@ class kernel_gen {
             w_.line("const unsigned long m0 = (tile % " + ul(MG_) + ") * " + ul(c_.Mb) + ";");
             w_.line("const unsigned long k0 = (tile / " + ul(MG_) + ") * " + ul(c_.Kb) + ";");
         }
-        w_.open("for (unsigned it = 0; it < " + u(iters) + "; ++it)");
-        w_.line("const unsigned cl = (wv + " + u(nwaves) + " * it) * 32u + cc;");
-        w_.line("const bool cv = cl < " + u(ncols) + ";");
-        w_.line("const unsigned mm = cl % " + u(c_.Mb) + ";");
-        w_.line("const unsigned jj = (cl / " + u(c_.Mb) + ") % " + u(N2) + ";");
-        w_.line("const unsigned kk = cl / " + u(c_.Mb * N2) + ";");
-        std::string ld = "cv && k0 + kk < kcols";
+        // All loads of the work-item are issued before the matrix instructions (one memory
+        // latency per tile instead of one per 32 columns).
+        std::string ld_cond = "cv && k0 + kk < kcols";
         if (MG_ * c_.Mb != c_.M) {
-            ld += " && m0 + mm < " + ul(c_.M);
+            ld_cond += " && m0 + mm < " + ul(c_.M);
         }
-        w_.line("const bool ld = " + ld + ";");
-        w_.line("const unsigned long bb = m0 + mm + (k0 + kk) * " + ul(c_.istride[2]) + ";");
-        for (unsigned h2 = 0; h2 < 2; ++h2) {
-            std::string r = "r" + std::to_string(h2);
-            w_.line("bbfft_u4 " + r + " = {0u, 0u, 0u, 0u};");
-            w_.open("if (ld)");
-            for (unsigned e = 0; e < 4; ++e) {
-                std::string n1 = "(" + u(8 * h2 + e) + " + 4u * hh)";
-                std::string off = "bb + (unsigned long)(jj + " + u(N2) + " * " + n1 + ") * " +
-                                  ul(c_.istride[1]);
-                w_.line(r + "[" + std::to_string(e) + "] = " + gload(off) + ";");
+        for (std::size_t it = 0; it < iters; ++it) {
+            std::string const x = "_" + std::to_string(it);
+            w_.line("const unsigned cl" + x + " = " + u((it * nwaves) * 32) +
+                    " + wv * 32u + cc;");
+            w_.line("const bool cv" + x + " = cl" + x + " < " + u(ncols) + ";");
+            w_.line("const unsigned mm" + x + " = cl" + x + " % " + u(c_.Mb) + ";");
+            w_.line("const unsigned jj" + x + " = (cl" + x + " / " + u(c_.Mb) + ") % " + u(N2) +
+                    ";");
+            w_.line("const unsigned kk" + x + " = cl" + x + " / " + u(c_.Mb * N2) + ";");
+            std::string ldx = ld_cond;
+            for (std::string v : {"cv", "kk", "mm"}) {
+                std::size_t pos = 0;
+                while ((pos = ldx.find(v, pos)) != std::string::npos) {
+                    ldx.insert(pos + v.size(), x);
+                    pos += v.size() + x.size();
+                }
+            }
+            w_.line("const bool ld" + x + " = " + ldx + ";");
+            w_.line("const unsigned long bb" + x + " = m0 + mm" + x + " + (k0 + kk" + x + ") * " +
+                    ul(c_.istride[2]) + ";");
+            for (unsigned h2 = 0; h2 < 2; ++h2) {
+                std::string r = "r" + std::to_string(h2) + x;
+                w_.line("bbfft_u4 " + r + " = {0u, 0u, 0u, 0u};");
+                w_.open("if (ld" + x + ")");
+                for (unsigned e = 0; e < 4; ++e) {
+                    std::string n1 = "(" + u(8 * h2 + e) + " + 4u * hh)";
+                    std::string off = "bb" + x + " + (unsigned long)(jj" + x + " + " + u(N2) +
+                                      " * " + n1 + ") * " + ul(c_.istride[1]);
+                    w_.line(r + "[" + std::to_string(e) + "] = " + gload(off) + ";");
+                }
+                w_.close();
+            }
+        }
+        for (std::size_t it = 0; it < iters; ++it) {
+            std::string const x = "_" + std::to_string(it);
+            w_.open("");
+            w_.line("bbfft_f16v acc = {};");
+            for (unsigned t = T; t-- > 0;) {
+                for (unsigned h2 = 0; h2 < 2; ++h2) {
+                    w_.line("acc = " + op + "(a" + std::to_string(t) + "_" + std::to_string(h2) +
+                            ", __builtin_bit_cast(" + vt + ", r" + std::to_string(h2) + x +
+                            "), acc, 0, 0, 0);");
+                }
+            }
+            w_.open("if (cv" + x + ")");
+            w_.line(T2() + "* cp = lds_buf + mm" + x + " + " + u(c_.Mb * np_) + " * kk" + x +
+                    ";");
+            for (unsigned r = 0; r < 16; r += 2) {
+                std::string k1 = "(" + u(4 * (r / 4) + (r % 4) / 2) + " + 2u * hh)";
+                w_.line(lds_at("cp", "jj" + x + " * 16u + " + k1) + " = mk(acc[" +
+                        std::to_string(r) + "], acc[" + std::to_string(r + 1) + "]);");
             }
             w_.close();
+            w_.close();
         }
-        w_.line("bbfft_f16v acc = {};");
-        for (unsigned t = T; t-- > 0;) {
-            for (unsigned h2 = 0; h2 < 2; ++h2) {
-                w_.line("acc = " + op + "(a" + std::to_string(t) + "_" + std::to_string(h2) +
-                        ", __builtin_bit_cast(" + vt + ", r" + std::to_string(h2) +
-                        "), acc, 0, 0, 0);");
-            }
-        }
-        w_.open("if (cv)");
-        w_.line(T2() + "* cp = lds_buf + mm + " + u(c_.Mb * np_) + " * kk;");
-        for (unsigned r = 0; r < 16; r += 2) {
-            std::string k1 = "(" + u(4 * (r / 4) + (r % 4) / 2) + " + 2u * hh)";
-            w_.line(lds_at("cp", "jj * 16u + " + k1) + " = mk(acc[" + std::to_string(r) +
-                    "], acc[" + std::to_string(r + 1) + "]);");
-        }
-        w_.close();
-        w_.close();
         w_.close();
         w_.line("__syncthreads();");
     }

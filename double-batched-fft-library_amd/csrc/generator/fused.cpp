@@ -11,6 +11,7 @@
 #include <iomanip>
 #include <ostream>
 #include <sstream>
+#include <vector>
 
 namespace bbfft {
 
@@ -25,8 +26,31 @@ std::uint64_t fnv1a64(std::string const &s) {
     return h;
 }
 
-/// Kernel configuration of pass `p` restricted to one plane (`kcols` K-columns, one tile),
-/// with the fewest work-items per transform that still fit `wg_max`; false if none fits.
+/// Factorisations of n into at most 3 radices in [2, rmax], largest radix first.
+void factorizations(std::size_t n, std::size_t rmax, std::size_t parts, std::vector<int> &cur,
+                    std::vector<std::vector<int>> &out) {
+    if (n == 1) {
+        if (!cur.empty()) {
+            out.push_back(cur);
+        }
+        return;
+    }
+    if (parts == 0) {
+        return;
+    }
+    std::size_t const hi = cur.empty() ? rmax : std::min<std::size_t>(rmax, cur.back());
+    for (std::size_t r = std::min(hi, n); r >= 2; --r) {
+        if (n % r == 0) {
+            cur.push_back(static_cast<int>(r));
+            factorizations(n / r, rmax, parts - 1, cur, out);
+            cur.pop_back();
+        }
+    }
+}
+
+/// Kernel configuration of pass `p` restricted to one plane (`kcols` K-columns, one tile):
+/// the factorisation with the smallest largest radix (fewest registers per work-item) whose
+/// work-group fits `wg_max`; false if none fits.
 bool one_tile(configuration p, std::size_t kcols, device_info const &info, unsigned flags,
               unsigned variant, std::size_t wg_max, fft1d_configuration &out) {
     p.shape[2] = kcols;
@@ -35,36 +59,43 @@ bool one_tile(configuration p, std::size_t kcols, device_info const &info, unsig
     std::size_t const N = p.shape[1];
     std::size_t const nfft = real && N % 2 == 0 ? N / 2 : N;
     std::size_t const cols = real && N % 2 == 1 ? (kcols + 1) / 2 : kcols;
-    int const fp_max = p.fp == precision::f32 ? 32 : 16;
-    for (int rmax : {8, 16, 32, 64}) {
-        if (rmax > fp_max && rmax != 64) {
-            continue;
-        }
-        auto f = gen::default_factorization(nfft, static_cast<std::size_t>(rmax),
-                                            static_cast<std::size_t>(rmax));
-        std::size_t const r = static_cast<std::size_t>(*std::max_element(f.begin(), f.end()));
-        if (r > 64) {
-            return false;
-        }
-        std::size_t const Nt = nfft / r;
-        if (M * Nt * cols > wg_max) {
-            continue;
-        }
-        fft1d_tuning t;
-        t.factorization = f;
-        t.Nt = Nt;
-        t.Mb = M;
-        t.Kb = cols;
-        t.flags = flags;
-        t.has_flags = true;
-        try {
-            out = configure_fft1d(p, info, t, variant);
-        } catch (bad_configuration const &) {
-            return false;
-        }
-        return true;
+    std::size_t const rmax = p.fp == precision::f32 ? 32 : 16;
+    std::vector<std::vector<int>> cands;
+    std::vector<int> cur;
+    if (nfft <= rmax) {
+        cands.push_back({static_cast<int>(nfft)});
     }
-    return false;
+    factorizations(nfft, rmax, 3, cur, cands);
+    std::vector<int> const *best = nullptr;
+    std::size_t best_key = ~std::size_t(0);
+    for (auto const &f : cands) {
+        std::size_t const r = static_cast<std::size_t>(f.front()); // largest radix
+        if (M * (nfft / r) * cols > wg_max) {
+            continue;
+        }
+        // smallest largest radix, then fewest stages, then most balanced
+        std::size_t const key = r * 64 + f.size() * 16 + (r - static_cast<std::size_t>(f.back()));
+        if (key < best_key) {
+            best_key = key;
+            best = &f;
+        }
+    }
+    if (!best) {
+        return false;
+    }
+    fft1d_tuning t;
+    t.factorization = *best;
+    t.Nt = nfft / static_cast<std::size_t>(best->front());
+    t.Mb = M;
+    t.Kb = cols;
+    t.flags = flags;
+    t.has_flags = true;
+    try {
+        out = configure_fft1d(p, info, t, variant);
+    } catch (bad_configuration const &) {
+        return false;
+    }
+    return true;
 }
 
 } // namespace
@@ -109,9 +140,10 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
     if (pa.shape[2] % ka || pb.shape[2] % kb || pa.shape[2] / ka != pb.shape[2] / kb) {
         return false;
     }
-    std::size_t const wg_max = std::min<std::size_t>(1024, info.max_work_group_size);
     fused2_configuration fc = {};
-    // streaming hints on the global side only
+    // As many work-items as the device allows, so each holds few elements (smallest radices);
+    // streaming hints on the global side only.
+    std::size_t const wg_max = std::min<std::size_t>(1024, info.max_work_group_size);
     if (!one_tile(pa, ka, info, fft1d_nontemporal_load, variant_a, wg_max, fc.a) ||
         !one_tile(pb, kb, info, fft1d_nontemporal_store, variant_b, wg_max, fc.b)) {
         return false;

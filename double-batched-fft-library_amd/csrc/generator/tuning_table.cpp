@@ -1,5 +1,6 @@
 // Persisted autotuning table: per (precision, transform type, N, M) the kernel parameters the
-// autotuner measured fastest on gfx950. Built into the library from tuning/gfx950.inc;
+// autotuner measured fastest on gfx950. The type may carry the input storage of half-input
+// kernels: c2c-f16, c2c-bf16, c2c-f16-mfma, c2c-bf16-mfma. Built into the library from tuning/gfx950.inc;
 // BBFFT_TUNING_FILE replaces it with a table file of the same format and BBFFT_TUNING=off
 // disables it (heuristics only).
 #include "tuning_table.hpp"
@@ -20,7 +21,10 @@ char const builtin_table[] =
 #include "../../tuning/gfx950.inc"
     ;
 
-using key_t = std::tuple<int, int, std::size_t, std::size_t>;
+using key_t = std::tuple<int, int, std::size_t, std::size_t, unsigned>;
+
+/// Input-storage bits of a kernel variant that select a separate table entry.
+constexpr unsigned storage_bits = fft1d_input_f16 | fft1d_input_bf16 | fft1d_mfma;
 
 std::map<key_t, fft1d_tuning> parse(std::string const &text) {
     std::map<key_t, fft1d_tuning> table;
@@ -44,6 +48,16 @@ std::map<key_t, fft1d_tuning> parse(std::string const &text) {
             continue;
         }
         int p = prec == "f32" ? 4 : prec == "f64" ? 8 : 0;
+        unsigned storage = 0;
+        for (auto [suffix, bit] : {std::pair<char const *, unsigned>{"-f16", fft1d_input_f16},
+                                   {"-bf16", fft1d_input_bf16},
+                                   {"-mfma", fft1d_mfma}}) {
+            auto pos = type.find(suffix);
+            if (pos != std::string::npos) {
+                storage |= bit;
+                type.erase(pos, std::string(suffix).size());
+            }
+        }
         int ty = type == "c2c" ? 0 : type == "r2c" ? 1 : type == "c2r" ? 2 : -1;
         if (!p || ty < 0) {
             continue;
@@ -55,7 +69,7 @@ std::map<key_t, fft1d_tuning> parse(std::string const &text) {
         }
         t.no_pad = t.lds_pad == 0;
         t.has_flags = true;
-        table[key_t{p, ty, N, M}] = t;
+        table[key_t{p, ty, N, M, storage}] = t;
     }
     return table;
 }
@@ -84,9 +98,10 @@ std::map<key_t, fft1d_tuning> const &table() {
 } // namespace
 
 bool lookup_tuning(precision fp, transform_type type, std::size_t N, std::size_t M,
-                   fft1d_tuning &out) {
+                   unsigned variant, fft1d_tuning &out) {
     auto const &t = table();
-    auto it = t.find(key_t{static_cast<int>(fp), static_cast<int>(type), N, M});
+    auto it = t.find(
+        key_t{static_cast<int>(fp), static_cast<int>(type), N, M, variant & storage_bits});
     if (it == t.end()) {
         return false;
     }

@@ -9,9 +9,11 @@
 
 namespace bbfft::gen {
 
-/// Tuned parameters for (precision, type, N, M), if the table has an entry.
+/// Tuned parameters for (precision, type, N, M) and the input storage bits of `variant`
+/// (half-precision input, MFMA stage), if the table has an entry. Only the access-hint flags
+/// are stored; the caller adds the variant bits.
 bool lookup_tuning(precision fp, transform_type type, std::size_t N, std::size_t M,
-                   fft1d_tuning &out);
+                   unsigned variant, fft1d_tuning &out);
 std::size_t tuning_table_size();
 
 } // namespace bbfft::gen

@@ -42,8 +42,15 @@ def test_table_entries_are_valid_configurations():
         fields = ln.split("#")[0].split(":")[1].split()
         fac = [int(v) for v in fields[0].split("x")]
         Nt, Mb, Kb, pad, flags = (int(v) for v in fields[1:6])
-        nfft = int(N) // 2 if typ != "c2c" and int(N) % 2 == 0 else int(N)
+        base, *storage = typ.split("-")
+        nfft = int(N) // 2 if base != "c2c" and int(N) % 2 == 0 else int(N)
         assert math.prod(fac) == nfft, ln
+        assert flags in (0, 1, 2, 3), ln  # access hints only; storage bits come from the type
+        if storage:  # half-input kernels: c2c fp32, MFMA entries start with radix 16
+            assert base == "c2c" and prec == "f32" and set(storage) <= {"f16", "bf16", "mfma"}
+            assert "mfma" not in storage or fac[0] == 16, ln
+            continue
+        typ = base
         desc = f"{'s' if prec == 'f32' else 'd'}{'c' if typ == 'c2c' else 'r'}" \
                f"{'b' if typ == 'c2r' else 'f'}o{M}.{N}*1000"
         kc = bb.kernel_configuration(bb.parse_fft_descriptor(desc))
