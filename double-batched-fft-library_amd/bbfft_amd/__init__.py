@@ -448,6 +448,22 @@ def half_input_flags(storage="f16", mfma=True) -> int:
     return (INPUT_F16 if storage == "f16" else INPUT_BF16) | (MFMA if mfma else 0)
 
 
+def half_input_kernel_configuration(cfg, storage="f16", mfma=True, device_info=GFX950_INFO):
+    """Kernel tiling of a half-input plan (tuning table applied), like kernel_configuration."""
+    if isinstance(cfg, str):
+        cfg = parse_fft_descriptor(cfg)
+    kc = _capi.CKernelConfig()
+    name = ctypes.create_string_buffer(512)
+    _check(_capi.load().bbfft_half_input_kernel_configuration(
+        ctypes.byref(cfg._to_c()), half_input_flags(storage, mfma), device_info.encode(),
+        ctypes.byref(kc), name, 512))
+    return {"M": kc.M, "Mb": kc.Mb, "N": kc.N, "K": kc.K, "Nt": kc.Nt, "Kb": kc.Kb,
+            "lds_pad": kc.lds_pad, "lds_bytes": kc.lds_bytes,
+            "work_group_size": kc.work_group_size,
+            "factorization": [kc.factorization[i] for i in range(kc.num_factors)],
+            "name": name.value.decode()}
+
+
 def make_half_input_plan(cfg, storage="f16", mfma=True, stream=None,
                          cache: Optional["Cache"] = None) -> "Plan":
     """1-D c2c fp32 plan whose input is stored as half-precision (re, im) pairs (4 bytes per

@@ -27,6 +27,7 @@ EXPORTED = [
     "bbfft_ragged_job_variants", "bbfft_ragged_job_table", "bbfft_generate_ragged_source",
     "bbfft_ragged_plan_create", "bbfft_aot_generate_ragged", "bbfft_plan_create_half_input",
     "bbfft_fused_configuration", "bbfft_generate_fused_source",
+    "bbfft_half_input_kernel_configuration",
 ]
 
 SUCCESS = 0
@@ -187,6 +188,8 @@ def load():
                                               c_char_p]),
         "bbfft_plan_create_half_input": (c_int, [P(CConfiguration), c_uint, c_void_p, c_void_p,
                                                  P(c_void_p)]),
+        "bbfft_half_input_kernel_configuration": (c_int, [P(CConfiguration), c_uint, c_char_p,
+                                                          P(CKernelConfig), c_char_p, c_size]),
         "bbfft_fused_configuration": (c_int, [P(CConfiguration), c_char_p, P(c_int),
                                               P(CFusedInfo), c_char_p, c_size]),
         "bbfft_generate_fused_source": (c_int, [P(CConfiguration), c_char_p, c_char_p, c_size,

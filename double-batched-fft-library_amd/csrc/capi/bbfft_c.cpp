@@ -532,6 +532,21 @@ int bbfft_generate_fused_source(const bbfft_configuration *cfg, const char *devi
     });
 }
 
+int bbfft_half_input_kernel_configuration(const bbfft_configuration *cfg, unsigned flags,
+                                          const char *device_info, bbfft_kernel_config *out,
+                                          char *name, size_t name_len) {
+    return guarded([&] {
+        require(cfg && device_info && out, "null argument");
+        unsigned const allowed = BBFFT_INPUT_F16 | BBFFT_INPUT_BF16 | BBFFT_MFMA;
+        require((flags & ~allowed) == 0 && (flags & (BBFFT_INPUT_F16 | BBFFT_INPUT_BF16)),
+                "invalid half-input flags");
+        auto layout = decompose(from_c(*cfg));
+        require(layout.passes.size() == 1, "half-input plans are 1-D");
+        auto kc = configure_fft1d(layout.passes[0], parse_device_info(device_info), {}, flags);
+        fill_kernel_config(kc, layout.passes[0], out, name, name_len);
+    });
+}
+
 int bbfft_plan_create_half_input(const bbfft_configuration *cfg, unsigned flags,
                                  bbfft_stream_t stream, bbfft_cache_t cache, bbfft_plan_t *plan) {
     return guarded([&] {

@@ -110,6 +110,7 @@ bool find_fused_pair(pass_layout const &layout, device_info const &info,
     if (env && std::string(env) == "off") {
         return false;
     }
+    bool const forced = env && std::string(env) == "on";
     std::size_t const D = layout.passes.size();
     if (D < 2) {
         return false;
@@ -124,9 +125,14 @@ bool find_fused_pair(pass_layout const &layout, device_info const &info,
     if (!configure_fused2(pa, pb, info, va, vb, fc)) {
         return false;
     }
-    // One work-group per plane: with few large planes the unfused passes spread the work over
-    // more work-groups; small problems are launch-bound and gain from one launch less.
-    if (fc.planes < 256 && fc.planes * fc.plane_elems > (std::size_t(1) << 20)) {
+    // One work-group per plane, which must overlap its memory latency with other work-groups
+    // on the same CU: planes above 48 KB of LDS (fewer than 3 resident work-groups; e.g. the
+    // 128 x 65 fp32 plane of 128^3 real transforms, measured slower fused than as two passes)
+    // and few large planes are left to the unfused passes, except for problems small enough to
+    // be launch-bound (one launch less). BBFFT_FUSE=on fuses whenever it fits.
+    std::size_t const total = fc.planes * fc.plane_elems;
+    bool const small = total <= (std::size_t(1) << 20);
+    if (!forced && !small && (fc.lds_bytes() > 48 * 1024 || fc.planes < 256)) {
         return false;
     }
     out = fused_pair{first, fc};

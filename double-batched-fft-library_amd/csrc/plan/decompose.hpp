@@ -45,7 +45,9 @@ struct fused_pair {
 /**
  * @brief The pair of passes of a 2-D/3-D layout to fuse on this device, if any: the first two
  * passes (modes 1 and 2) for c2c and r2c, the last two for c2r (the passes run in reverse
- * mode order). `real_pass_variant` applies to the real pass. BBFFT_FUSE=off disables fusion.
+ * mode order), when the plane is small enough for several work-groups per CU or the whole
+ * problem is launch-bound. `real_pass_variant` applies to the real pass. BBFFT_FUSE=off
+ * disables fusion, BBFFT_FUSE=on applies it whenever the plane fits.
  */
 bool find_fused_pair(pass_layout const &layout, device_info const &info,
                      unsigned real_pass_variant, fused_pair &out);

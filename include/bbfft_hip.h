@@ -182,9 +182,10 @@ BBFFT_EXPORT_C int bbfft_device_info_query(int device, char *buf, size_t len);
 /*
  * Fused passes of 2-D/3-D plans (extension of the reference's nd_fft,
  * src/common/algorithm/nd_fft.hpp:52-99, which runs one memory round trip per mode): where a
- * plane of the intermediate tensor fits in LDS, the passes over modes 1 and 2 run as one
- * kernel. These entry points report the decision and the kernel for a device_info (host only);
- * plans apply it automatically (BBFFT_FUSE=off in the environment disables it).
+ * plane of the intermediate tensor fits in LDS (and is small enough for several work-groups
+ * per CU, or the problem is launch-bound), the passes over modes 1 and 2 run as one kernel.
+ * These entry points report the decision and the kernel for a device_info (host only); plans
+ * apply it automatically (BBFFT_FUSE=off disables it, BBFFT_FUSE=on forces it where it fits).
  */
 typedef struct bbfft_fused_info {
     unsigned first_pass;
@@ -210,6 +211,13 @@ BBFFT_EXPORT_C int bbfft_generate_fused_source(const bbfft_configuration *cfg,
  */
 enum { BBFFT_INPUT_F16 = 8, BBFFT_INPUT_BF16 = 16, BBFFT_MFMA = 32 };
 
+/* Tiling a half-input plan uses (the tuning table included): `flags` as for
+ * bbfft_plan_create_half_input. */
+BBFFT_EXPORT_C int bbfft_half_input_kernel_configuration(const bbfft_configuration *cfg,
+                                                         unsigned flags,
+                                                         const char *device_info,
+                                                         bbfft_kernel_config *out, char *name,
+                                                         size_t name_len);
 BBFFT_EXPORT_C int bbfft_plan_create_half_input(const bbfft_configuration *cfg, unsigned flags,
                                                 bbfft_stream_t stream, bbfft_cache_t cache,
                                                 bbfft_plan_t *plan);

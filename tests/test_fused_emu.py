@@ -80,17 +80,22 @@ def test_fused_in_place_layout_r2c():
     assert oracle.rel_l2(y[sel], ref[sel]) <= oracle.tolerance(4, 512)
 
 
-def test_fusion_policy():
-    """The 3-D 32 x 128^3 real transforms fuse modes 1 and 2 (one 128 x 65 plane per work-group,
-    plane and stage scratch sharing 70 KB of LDS); 2-D planes that do not fit are not fused;
-    few large planes are left to the unfused passes."""
-    fc = bb.fused_configuration(bb.parse_fft_descriptor("srfo128x128x128*32"))
+def test_fusion_policy(monkeypatch):
+    """Planes of at most 48 KB of LDS fuse (several work-groups per CU); the 128 x 65 fp32
+    plane of the 32 x 128^3 real transforms (70 KB) is left to the unfused passes unless
+    BBFFT_FUSE=on; planes that do not fit never fuse; small problems fuse (launch-bound)."""
+    c3 = bb.parse_fft_descriptor("srfo128x128x128*32")
+    c2r = _cfg(1, [128, 128, 128], 32, 4, 2)
+    assert bb.fused_configuration(c3) is None
+    assert bb.fused_configuration(bb.parse_fft_descriptor("scfo64x64x64*16")) is not None
+    assert bb.fused_configuration(bb.parse_fft_descriptor("scfo256x256*4")) is None
+    assert bb.fused_configuration(bb.parse_fft_descriptor("scfo64x64*100")) is not None
+    monkeypatch.setenv("BBFFT_FUSE", "on")
+    fc = bb.fused_configuration(c3)
     assert fc["first_pass"] == 0 and fc["planes"] == 128 * 32 and fc["plane_elems"] == 65 * 128
     assert fc["lds_bytes"] <= 80 * 1024 and fc["group_size"] <= 1024
-    c2r = _cfg(1, [128, 128, 128], 32, 4, 2)
     fr = bb.fused_configuration(c2r)
     assert fr["first_pass"] == 1 and fr["kb"] == 128 and fr["ka"] == 1
     assert bb.fused_configuration(bb.parse_fft_descriptor("scfo256x256*4")) is None
-    assert bb.fused_configuration(bb.parse_fft_descriptor("scfo128x128*300")) is not None
-    assert bb.fused_configuration(bb.parse_fft_descriptor("scfo128x128*100")) is None
-    assert bb.fused_configuration(bb.parse_fft_descriptor("scfo64x64*100")) is not None
+    monkeypatch.setenv("BBFFT_FUSE", "off")
+    assert bb.fused_configuration(bb.parse_fft_descriptor("scfo64x64*100")) is None

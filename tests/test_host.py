@@ -266,6 +266,13 @@ def test_tuning_table_is_applied():
     entries = _table_entries()
     assert len(entries) >= 20
     for prec, typ, N, M, fac, Nt, Mb, Kb, pad, flags in entries:
+        if "-" in typ:  # half-input kernels: c2c-{f16,bf16}[-mfma]
+            _, storage, *mfma = typ.split("-")
+            kc = bb.half_input_kernel_configuration(f"scfo{M}.{N}*65536", storage, bool(mfma))
+            assert (kc["factorization"], kc["Nt"], kc["Mb"], kc["lds_pad"]) == \
+                (fac, Nt, Mb, pad), typ
+            assert kc["Kb"] >= Kb and kc["work_group_size"] % 64 == 0
+            continue
         d = {"c2c": "c", "r2c": "r", "c2r": "r"}[typ]
         dirn = "b" if typ == "c2r" else "f"
         desc = f"{prec[0].replace('f', 's') if prec == 'f32' else 'd'}{d}{dirn}o{M}.{N}*65536"
