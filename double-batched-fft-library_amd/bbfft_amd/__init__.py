@@ -213,6 +213,7 @@ def tuning_struct(t):
     for k in ("Nt", "Mb", "Kb", "lds_pad"):
         setattr(c, k, int(t.get(k, 0)))
     c.no_pad = int(bool(t.get("no_pad", False)))
+    c.waves_per_eu = int(t.get("waves_per_eu", 0))
     if "flags" in t:
         c.flags = int(t["flags"])
         c.has_flags = 1

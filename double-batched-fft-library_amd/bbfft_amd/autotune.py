@@ -24,7 +24,7 @@ from . import (GFX950_INFO, BadConfiguration, BbfftError, default_cache, kernel_
 
 TABLE_HEADER = ("# Tuned kernel parameters for gfx950 (MI355X), written by "
                 "`python -m bbfft_amd.autotune`.\n"
-                "# precision type N M : factorization Nt Mb Kb lds_pad flags"
+                "# precision type N M : factorization Nt Mb Kb lds_pad flags [waves_per_eu]"
                 "   # GB/s default -> tuned, descriptor\n")
 
 
@@ -170,7 +170,7 @@ class Tuner:
         kc = kernel_configuration(self.cfg, tuning=partial, device_info=GFX950_INFO)
         return {"factorization": kc["factorization"], "Nt": kc["Nt"], "Mb": kc["Mb"],
                 "Kb": kc["Kb"], "lds_pad": kc["lds_pad"], "no_pad": kc["lds_pad"] == 0,
-                "flags": partial["flags"]}
+                "flags": partial["flags"], "waves_per_eu": partial.get("waves_per_eu", 0)}
 
     def factorizations(self):
         if self.variant & 32:  # the MFMA stage is a first radix of 16
@@ -215,6 +215,9 @@ class Tuner:
             # 5) streaming hints
             opts = [dict(best, flags=fl | self.variant) for fl in (0, 1, 2, 3)]
             best, best_ms = self._try(best, best_ms, opts)
+            # 6) occupancy target of the register allocator
+            opts = [dict(best, waves_per_eu=w) for w in (0, 4, 5, 6, 8)]
+            best, best_ms = self._try(best, best_ms, opts)
         return best, best_ms
 
     def _try(self, best, best_ms, options, complete=False):
@@ -238,8 +241,9 @@ def table_line(cfg, tuning, default_ms, best_ms, nbytes, desc):
     if flags & 32:
         typ += "-mfma"
     fac = "x".join(str(f) for f in tuning["factorization"])
+    waves = tuning.get("waves_per_eu", 0)
     return (f"{prec} {typ} {cfg.shape[1]} {cfg.shape[0]} : {fac} {tuning['Nt']} {tuning['Mb']} "
-            f"{tuning['Kb']} {tuning['lds_pad']} {flags & 3}   # "
+            f"{tuning['Kb']} {tuning['lds_pad']} {flags & 3}{f' {waves}' if waves else ''}   # "
             f"{nbytes / default_ms / 1e6:.0f} -> {nbytes / best_ms / 1e6:.0f} GB/s, {desc}")
 
 

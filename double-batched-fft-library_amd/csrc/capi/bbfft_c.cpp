@@ -150,6 +150,7 @@ fft1d_tuning tuning_of(bbfft_tuning const *t) {
     r.no_pad = t->no_pad != 0;
     r.flags = t->flags;
     r.has_flags = t->has_flags != 0;
+    r.waves_per_eu = t->waves_per_eu;
     return r;
 }
 

@@ -55,6 +55,9 @@ std::string fft1d_configuration::identifier() const {
     if (flags) {
         oss << "_g" << flags;
     }
+    if (waves_per_eu) {
+        oss << "_w" << waves_per_eu;
+    }
     if (load_function) {
         oss << '_' << load_function;
     }
@@ -255,6 +258,10 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
         throw bad_configuration("Work-group exceeds the local memory size");
     }
     c.flags = (tuning.has_flags ? tuning.flags : 0u) | variant;
+    if (tuning.waves_per_eu > 8) {
+        throw bad_configuration("waves_per_eu must be 0..8");
+    }
+    c.waves_per_eu = tuning.waves_per_eu;
     if ((c.flags & fft1d_adjoint) && cfg.type == transform_type::c2c) {
         throw bad_configuration("The adjoint variant applies to real transforms only");
     }

@@ -490,7 +490,12 @@ class kernel_gen {
     // ---- kernel ----------------------------------------------------------------------------
     void emit_kernel() {
         std::size_t const wg = c_.work_group_size();
-        w_.line("extern \"C\" __global__ void __launch_bounds__(" + std::to_string(wg) + ") " +
+        std::string const occupancy =
+            c_.waves_per_eu ? " __attribute__((amdgpu_waves_per_eu(" +
+                                  std::to_string(c_.waves_per_eu) + ")))"
+                            : "";
+        w_.line("extern \"C\" __global__ void __launch_bounds__(" + std::to_string(wg) + ")" +
+                occupancy + " " +
                 name_ + "(const " + in_type() + "* in, " + out_type() +
                 "* out, unsigned long K) {");
         w_.open("");

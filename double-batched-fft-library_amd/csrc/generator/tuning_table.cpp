@@ -67,6 +67,10 @@ std::map<key_t, fft1d_tuning> parse(std::string const &text) {
         while (std::getline(fs, f, 'x')) {
             t.factorization.push_back(std::stoi(f));
         }
+        unsigned waves = 0;
+        if (rhs >> waves) { // optional occupancy target
+            t.waves_per_eu = waves;
+        }
         t.no_pad = t.lds_pad == 0;
         t.has_flags = true;
         table[key_t{p, ty, N, M, storage}] = t;

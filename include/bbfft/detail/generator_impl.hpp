@@ -43,6 +43,7 @@ struct BBFFT_EXPORT fft1d_configuration {
     std::array<std::size_t, 3u> ostride; ///< {1, s_N, s_K} of the output
     std::size_t lds_pad;                 ///< one padding slot per lds_pad slots (0 = none)
     unsigned flags;                      ///< fft1d_flag bits (global access hints)
+    unsigned waves_per_eu;               ///< occupancy target of the compiler (0 = default)
     bool inplace_unsupported;            ///< in == out would race across work-groups
     char const *load_function;           ///< user load callback or nullptr
     char const *store_function;          ///< user store callback or nullptr
@@ -96,6 +97,9 @@ struct BBFFT_EXPORT fft1d_tuning {
     bool no_pad = false;
     unsigned flags = 0;
     bool has_flags = false; ///< use `flags` instead of the default
+    /// Waves per SIMD the register allocator must allow (0 = its default); a higher target
+    /// trades registers (possibly spills) for occupancy.
+    unsigned waves_per_eu = 0;
 };
 
 /// Choose a kernel configuration for a 1-D configuration (dim == 1). `variant` (fft1d_flag

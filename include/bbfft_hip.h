@@ -76,6 +76,7 @@ typedef struct bbfft_tuning {
     int no_pad;
     unsigned flags;
     int has_flags;
+    unsigned waves_per_eu; /* occupancy target of the compiler, 0..8 (0 = its default) */
 } bbfft_tuning;
 
 /* Message of the last failed call on this thread ("" if none). */

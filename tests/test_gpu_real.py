@@ -145,3 +145,41 @@ def test_3d_r2c_c2r_128cubed(desc):
         y = execute(cfg, Xr.ravel())
         ref = np.fft.irfftn(Xr, s=(128, 128, 128), axes=(1, 2, 3)).ravel() * 128 ** 3
     assert oracle.rel_l2(y, ref) <= oracle.tolerance(4, 128 ** 3)
+
+
+@pytest.mark.parametrize("desc", [d for d in gpu_cases.c3_descriptors() if d.endswith("*2")] +
+                         ["scfo64x64x64*4", "scbo96x80*40", "drfo48x40x8*3", "drbo48x40x8*3"])
+def test_fused_passes_forced(desc, monkeypatch):
+    """Plans with the passes over modes 1 and 2 fused (BBFFT_FUSE=on, one plane per
+    work-group in LDS) against the unfused plan (BBFFT_FUSE=off) and numpy's pocketfft."""
+    cfg = bb.parse_fft_descriptor(desc)
+    monkeypatch.setenv("BBFFT_FUSE", "on")
+    fc = bb.fused_configuration(cfg)
+    assert fc is not None, desc
+    fused = bb.make_plan(cfg, cache=bb.default_cache())
+    assert any(l["kernel"] == fc["name"] for l in fused.launches())
+    monkeypatch.setenv("BBFFT_FUSE", "off")
+    plain = bb.make_plan(cfg, cache=bb.default_cache())
+    assert all("fused" not in l["kernel"] for l in plain.launches())
+    D = cfg.dim
+    Ns = list(cfg.shape[1:D + 1])
+    B = cfg.shape[D + 1]
+    t = int(cfg.type)
+    rng = np.random.default_rng(11)
+    axes = tuple(range(1, D + 1))
+    xr = rng.standard_normal([B] + Ns[::-1])
+    if t == 0:
+        x = xr + 1j * rng.standard_normal(xr.shape)
+        ref = (np.fft.fftn(x, axes=axes) if int(cfg.dir) < 0 else
+               np.fft.ifftn(x, axes=axes) * np.prod(Ns)).ravel()
+        inp = x.ravel()
+    elif t == 1:
+        inp, ref = xr.ravel(), np.fft.rfftn(xr, axes=axes).ravel()
+    else:
+        X = np.fft.rfftn(xr, axes=axes)
+        inp, ref = X.ravel(), xr.ravel() * np.prod(Ns)
+    y_f = execute(cfg, inp, plan=fused)
+    y_p = execute(cfg, inp, plan=plain)
+    tol = oracle.tolerance(int(cfg.fp), int(np.prod(Ns)))
+    assert oracle.rel_l2(y_f, ref) <= tol
+    assert oracle.rel_l2(y_f, y_p) <= tol
