@@ -27,7 +27,8 @@ EXPORTED = [
     "bbfft_ragged_job_variants", "bbfft_ragged_job_table", "bbfft_generate_ragged_source",
     "bbfft_ragged_plan_create", "bbfft_aot_generate_ragged", "bbfft_plan_create_half_input",
     "bbfft_fused_configuration", "bbfft_generate_fused_source",
-    "bbfft_half_input_kernel_configuration",
+    "bbfft_half_input_kernel_configuration", "bbfft_cache_load_aot_memory",
+    "bbfft_aot_archive_kernels",
 ]
 
 SUCCESS = 0
@@ -159,6 +160,8 @@ def load():
         "bbfft_cache_create": (c_int, [P(c_void_p)]),
         "bbfft_cache_load_aot": (c_int, [c_void_p, c_char_p, c_int, P(c_size)]),
         "bbfft_cache_size": (c_int, [c_void_p, P(c_size)]),
+        "bbfft_cache_load_aot_memory": (c_int, [c_void_p, c_void_p, c_size, c_int, P(c_size)]),
+        "bbfft_aot_archive_kernels": (c_int, [c_void_p, c_size, c_char_p, c_size, P(c_size)]),
         "bbfft_cache_destroy": (None, [c_void_p]),
         "bbfft_plan_create": (c_int, [P(CConfiguration), c_void_p, c_void_p, P(c_void_p)]),
         "bbfft_plan_create_from_descriptor": (c_int, [c_char_p, c_void_p, c_void_p,

@@ -48,6 +48,47 @@ def generate(descriptors, out_dir, device_info=None, arch="gfx950", jobs=8,
                   if f.startswith("kernels_"))
 
 
+def embed(archives, out_cpp, symbol="bbfft_aot_archive"):
+    """Write a C++ source holding the archives (concatenated entries) as a byte array and a
+    function `std::size_t <symbol>_register(bbfft::aot_cache &, int device)` registering them:
+    link it into an executable to ship its kernels without archive files."""
+    import re
+    import struct
+    if not re.fullmatch(r"[A-Za-z_][A-Za-z0-9_]*", symbol):
+        raise ValueError("symbol must be a C identifier")
+    entries, count = b"", 0
+    for path in archives:
+        with open(path, "rb") as f:
+            data = f.read()
+        if data[:8] != b"BBFFTAOT" or struct.unpack_from("<I", data, 8)[0] != 1:
+            raise ValueError(f"not a version-1 AOT archive: {path}")
+        count += struct.unpack_from("<I", data, 12)[0]
+        entries += data[16:]
+    blob = b"BBFFTAOT" + struct.pack("<II", 1, count) + entries
+    lines = []
+    for i in range(0, len(blob), 24):
+        lines.append("    " + ", ".join(str(b) for b in blob[i:i + 24]) + ",")
+    with open(out_cpp, "w") as f:
+        f.write(f"""// Ahead-of-time FFT kernels for gfx950 linked into the executable (written by
+// `python -m bbfft_amd.aot --embed`). Register them with {symbol}_register(cache, device).
+#include "bbfft/aot_cache.hpp"
+#include "bbfft/hip/online_compiler.hpp"
+
+#include <cstddef>
+#include <cstdint>
+
+alignas(16) extern const std::uint8_t {symbol}[] = {{
+""" + "\n".join(lines) + f"""
+}};
+extern const std::size_t {symbol}_size = {len(blob)};
+
+std::size_t {symbol}_register(bbfft::aot_cache &cache, int device) {{
+    return bbfft::hip::register_aot_archive(cache, {symbol}, {symbol}_size, device);
+}}
+""")
+    return len(blob)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("-o", "--out-dir")
@@ -57,10 +98,21 @@ def main(argv=None):
     ap.add_argument("-j", "--jobs", type=int, default=8)
     ap.add_argument("--shard", type=int, default=None)
     ap.add_argument("--num-shards", type=int, default=1)
+    ap.add_argument("--embed", metavar="OUT_CPP",
+                    help="write the archives given as arguments (or OUT_DIR/kernels_*.bin) "
+                         "into a C++ source to link into an executable")
+    ap.add_argument("--symbol", default="bbfft_aot_archive", help="array name for --embed")
     ap.add_argument("--ragged-serving", action="store_true",
                     help="compile the ragged kernel of the seeded serving workload to --out-file")
     ap.add_argument("descriptors", nargs="*")
     a = ap.parse_args(argv)
+    if a.embed:
+        files = a.descriptors or sorted(
+            os.path.join(a.out_dir, f) for f in os.listdir(a.out_dir)
+            if f.startswith("kernels_") and f.endswith(".bin"))
+        n = embed(files, a.embed, a.symbol)
+        print(f"{len(files)} archives ({n} bytes) -> {a.embed}", flush=True)
+        return 0
     if a.ragged_serving:
         from . import GFX950_INFO, ragged
         ragged.aot_generate(ragged.synthetic_serving_jobs(), a.out_file,

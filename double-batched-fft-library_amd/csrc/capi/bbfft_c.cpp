@@ -413,6 +413,40 @@ int bbfft_cache_load_aot(bbfft_cache_t cache, const char *path, int device, size
     });
 }
 
+int bbfft_cache_load_aot_memory(bbfft_cache_t cache, const void *data, size_t size, int device,
+                                size_t *num) {
+    return guarded([&] {
+        require(cache && data, "null argument");
+        auto n = hip::register_aot_archive(cache->aot, static_cast<std::uint8_t const *>(data),
+                                           size, device);
+        cache->aot_kernels += n;
+        if (num) {
+            *num = n;
+        }
+    });
+}
+
+int bbfft_aot_archive_kernels(const void *data, size_t size, char *buf, size_t len,
+                              size_t *needed) {
+    return guarded([&] {
+        require(data != nullptr, "null argument");
+        auto entries =
+            hip::parse_archive(static_cast<std::uint8_t const *>(data), size, "(memory)");
+        std::string s;
+        for (auto const &e : entries) {
+            for (auto const &n : e.kernel_names) {
+                s += e.arch + " " + n + "\n";
+            }
+        }
+        if (needed) {
+            *needed = s.size() + 1;
+        }
+        if (buf) {
+            copy_out(s, buf, len);
+        }
+    });
+}
+
 int bbfft_cache_size(bbfft_cache_t cache, size_t *num) {
     return guarded([&] {
         require(cache && num, "null argument");

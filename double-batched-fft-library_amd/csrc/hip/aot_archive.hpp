@@ -20,6 +20,9 @@ struct archive_entry {
 
 void write_archive(std::string const &path, std::vector<archive_entry> const &entries);
 std::vector<archive_entry> read_archive(std::string const &path);
+/// Parse an archive held in memory (`what` names it in error messages).
+std::vector<archive_entry> parse_archive(std::uint8_t const *data, std::size_t size,
+                                         std::string const &what);
 
 } // namespace bbfft::hip
 

@@ -177,8 +177,9 @@ aot_module create_aot_module(std::uint8_t const *binary, std::size_t binary_size
     return m;
 }
 
-std::size_t register_aot_archive(aot_cache &cache, std::string const &path, int device) {
-    auto entries = read_archive(path);
+namespace {
+std::size_t register_entries(aot_cache &cache, std::vector<archive_entry> const &entries,
+                             std::string const &path, int device) {
     std::string const arch = get_device_arch(device);
     std::size_t n = 0;
     for (auto const &e : entries) {
@@ -192,6 +193,16 @@ std::size_t register_aot_archive(aot_cache &cache, std::string const &path, int 
     }
     return n;
 }
+} // namespace
+
+std::size_t register_aot_archive(aot_cache &cache, std::string const &path, int device) {
+    return register_entries(cache, read_archive(path), path, device);
+}
+
+std::size_t register_aot_archive(aot_cache &cache, std::uint8_t const *data, std::size_t size,
+                                 int device) {
+    return register_entries(cache, parse_archive(data, size, "(memory)"), "(memory)", device);
+}
 
 // ---- archive ------------------------------------------------------------------------------
 
@@ -203,32 +214,33 @@ void put_str(std::ostream &os, std::string const &s) {
     os.write(s.data(), static_cast<std::streamsize>(s.size()));
 }
 struct reader {
-    std::vector<std::uint8_t> const &d;
+    std::uint8_t const *d;
+    std::size_t size;
     std::size_t p = 0;
     std::string const &path;
     void need(std::size_t n) {
-        if (p + n > d.size()) {
+        if (n > size || p > size - n) {
             throw error("Truncated AOT archive " + path, 0);
         }
     }
     std::uint32_t u32() {
         need(4);
         std::uint32_t v;
-        std::memcpy(&v, d.data() + p, 4);
+        std::memcpy(&v, d + p, 4);
         p += 4;
         return v;
     }
     std::uint64_t u64() {
         need(8);
         std::uint64_t v;
-        std::memcpy(&v, d.data() + p, 8);
+        std::memcpy(&v, d + p, 8);
         p += 8;
         return v;
     }
     std::string str() {
         auto n = u32();
         need(n);
-        std::string s(reinterpret_cast<char const *>(d.data() + p), n);
+        std::string s(reinterpret_cast<char const *>(d + p), n);
         p += n;
         return s;
     }
@@ -257,16 +269,22 @@ void write_archive(std::string const &path, std::vector<archive_entry> const &en
 
 std::vector<archive_entry> read_archive(std::string const &path) {
     auto data = read_file(path);
-    if (data.size() < 16 || std::memcmp(data.data(), "BBFFTAOT", 8) != 0) {
-        throw error("Not an AOT archive: " + path, 0);
+    return parse_archive(data.data(), data.size(), path);
+}
+
+std::vector<archive_entry> parse_archive(std::uint8_t const *data, std::size_t size,
+                                         std::string const &what) {
+    if (!data || size < 16 || std::memcmp(data, "BBFFTAOT", 8) != 0) {
+        throw error("Not an AOT archive: " + what, 0);
     }
-    reader r{data, 8, path};
+    reader r{data, size, 8, what};
     if (r.u32() != 1) {
-        throw error("Unsupported AOT archive version: " + path, 0);
+        throw error("Unsupported AOT archive version: " + what, 0);
     }
     auto count = r.u32();
-    std::vector<archive_entry> out(count);
-    for (auto &e : out) {
+    std::vector<archive_entry> out;
+    for (std::uint32_t c = 0; c < count; ++c) {
+        archive_entry e;
         e.arch = r.str();
         auto nn = r.u32();
         for (std::uint32_t i = 0; i < nn; ++i) {
@@ -274,9 +292,9 @@ std::vector<archive_entry> read_archive(std::string const &path) {
         }
         auto sz = r.u64();
         r.need(sz);
-        e.code.assign(data.begin() + static_cast<std::ptrdiff_t>(r.p),
-                      data.begin() + static_cast<std::ptrdiff_t>(r.p + sz));
+        e.code.assign(data + r.p, data + r.p + sz);
         r.p += sz;
+        out.push_back(std::move(e));
     }
     return out;
 }

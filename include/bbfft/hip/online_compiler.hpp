@@ -42,6 +42,11 @@ BBFFT_EXPORT aot_module create_aot_module(std::uint8_t const *binary, std::size_
  */
 BBFFT_EXPORT std::size_t register_aot_archive(aot_cache &cache, std::string const &path,
                                               int device);
+/// Same for an archive held in memory (e.g. linked into the executable, see
+/// `python -m bbfft_amd.aot --embed`; the reference links archives with `ld -r -b binary`,
+/// cmake/AddAotKernelsToTarget.cmake:4-33).
+BBFFT_EXPORT std::size_t register_aot_archive(aot_cache &cache, std::uint8_t const *data,
+                                              std::size_t size, int device);
 
 } // namespace bbfft::hip
 

@@ -142,6 +142,14 @@ BBFFT_EXPORT_C int bbfft_compile_kernel(const bbfft_configuration *cfg, unsigned
 BBFFT_EXPORT_C int bbfft_cache_create(bbfft_cache_t *cache);
 BBFFT_EXPORT_C int bbfft_cache_load_aot(bbfft_cache_t cache, const char *path, int device,
                                         size_t *num_kernels);
+/* Register an archive held in memory (linked into the executable: `python -m bbfft_amd.aot
+ * --embed` writes it as a C++ array; the reference links archives with `ld -r -b binary`,
+ * cmake/AddAotKernelsToTarget.cmake:4-33). */
+BBFFT_EXPORT_C int bbfft_cache_load_aot_memory(bbfft_cache_t cache, const void *data, size_t size,
+                                               int device, size_t *num_kernels);
+/* Host-only listing of an archive in memory: one "arch kernel_name" line per kernel. */
+BBFFT_EXPORT_C int bbfft_aot_archive_kernels(const void *data, size_t size, char *buf, size_t len,
+                                             size_t *needed);
 BBFFT_EXPORT_C int bbfft_cache_size(bbfft_cache_t cache, size_t *num_modules);
 BBFFT_EXPORT_C void bbfft_cache_destroy(bbfft_cache_t cache);
 
