@@ -70,3 +70,18 @@ def test_merge_and_line_format(tmp_path):
     t = {"factorization": [8, 8, 4], "Nt": 32, "Mb": 8, "Kb": 1, "lds_pad": 16, "flags": 3}
     line = autotune.table_line(cfg, t, 1.0, 0.5, 1e9, "scfo8.256*32768")
     assert re.match(r"f32 c2c 256 8 : 8x8x4 32 8 1 16 3 +# 1000 -> 2000 GB/s", line)
+
+
+def test_benchmark_cli_k_selection():
+    """K fixed (-k) or from the input tensor's memory footprint (-b), reference
+    benchmark/test.hpp:9-34, and the byte-size suffixes of benchmark/args.cpp."""
+    from bbfft_amd import benchmark
+    assert benchmark.parse_bytes("1G") == 1 << 30 and benchmark.parse_bytes("4k") == 4096
+    assert benchmark.parse_bytes("123") == 123
+
+    class A:
+        k, b = 0, 1 << 30
+    assert benchmark.k_for(A, 8, 256, 4, True) == (1 << 30) // (8 * 256 * 8)
+    assert benchmark.k_for(A, 1, 1 << 28, 8, True) == 1
+    A.k = 77
+    assert benchmark.k_for(A, 8, 256, 4, True) == 77
