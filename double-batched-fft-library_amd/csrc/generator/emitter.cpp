@@ -335,6 +335,15 @@ class kernel_gen {
             }
         }
         np_ = (c.lds_pad ? (nfft_ - 1) + (nfft_ - 1) / c.lds_pad : nfft_ - 1) + 1;
+        // 32-bit offsets within a tile when no callback needs absolute offsets and the tile's
+        // extent fits comfortably
+        std::size_t const kspan = (pairs_ ? 2 : 1) * c.Kb + 1;
+        std::size_t const nspan = c.N + 2;
+        std::size_t const ext_in = c.Mb + nspan * c.istride[1] + kspan * c.istride[2];
+        std::size_t const ext_out = c.Mb + nspan * c.ostride[1] + kspan * c.ostride[2];
+        std::size_t const eb = 2 * static_cast<std::size_t>(c.fp); // largest element size
+        local32_ = !c.load_function && !c.store_function &&
+                   ext_in * eb < (std::size_t(1) << 31) && ext_out * eb < (std::size_t(1) << 31);
         half_in_ = (c.flags & (fft1d_input_f16 | fft1d_input_bf16)) != 0;
         bf16_ = (c.flags & fft1d_input_bf16) != 0;
         mfma_ = (c.flags & fft1d_mfma) != 0;
@@ -363,12 +372,32 @@ class kernel_gen {
     std::string T() const { return f_.real(); }
     std::string T2() const { return f_.cplx(); }
 
+    /// Padded LDS index of FFT index n (expression).
+    std::string padded(std::string const &n) const {
+        return c_.lds_pad ? "(" + n + " + ((" + n + ") >> " + std::to_string(pad_shift_) + "))"
+                          : "(" + n + ")";
+    }
+    /// LDS slot at padded index `pbase` (a variable holding a padded index) plus a constant
+    /// padded offset: the offset folds into the LDS instruction's immediate.
+    std::string lds_slot(std::string const &pbase, std::size_t offset) const {
+        return "col[(" + pbase + " + " + u(offset) + ") * " + u(c_.Mb) + "]";
+    }
+
     /// LDS slot of FFT index n (expression) within this work-item's column.
     std::string lds(std::string const &n) const {
         std::string p = c_.lds_pad ? "(" + n + " + ((" + n + ") >> " +
                                          std::to_string(pad_shift_) + "))"
                                    : "(" + n + ")";
         return "col[" + p + " * " + u(c_.Mb) + "]";
+    }
+
+    std::size_t in_elem_bytes() const {
+        std::size_t const rb = static_cast<std::size_t>(c_.fp);
+        return half_in_ ? 4 : c_.type == transform_type::r2c ? rb : 2 * rb;
+    }
+    std::size_t out_elem_bytes() const {
+        std::size_t const rb = static_cast<std::size_t>(c_.fp);
+        return c_.type == transform_type::c2r ? rb : 2 * rb;
     }
 
     std::string in_type() const {
@@ -387,9 +416,9 @@ class kernel_gen {
             return std::string(c_.load_function) + "((" + in_type() + "*)in, " + off + ")";
         }
         if (c_.flags & fft1d_nontemporal_load) {
-            return "ld_nt(in + " + off + ")";
+            return "ld_nt(tin + " + off + ")";
         }
-        return "in[" + off + "]";
+        return "tin[" + off + "]";
     }
     /// Global store of `val` at offset `off` (elements from `out`).
     std::string gstore(std::string const &off, std::string const &val) const {
@@ -397,9 +426,9 @@ class kernel_gen {
             return std::string(c_.store_function) + "(out, " + off + ", " + val + ");";
         }
         if (c_.flags & fft1d_nontemporal_store) {
-            return "st_nt(out + " + off + ", " + val + ");";
+            return "st_nt(tout + " + off + ", " + val + ");";
         }
-        return "out[" + off + "] = " + val + ";";
+        return "tout[" + off + "] = " + val + ";";
     }
 
     std::string tw_name(std::size_t s) const { return name_ + "_tw" + std::to_string(s); }
@@ -555,14 +584,16 @@ class kernel_gen {
 
     /// One (M tile, K tile) of work: `tile` selects it.
     void emit_tile() {
+        // tile origin (uniform across the work-group)
         if (MG_ == 1) {
-            w_.line("const unsigned long m = ml;");
-            w_.line("const unsigned long kc = tile * " + ul(c_.Kb) + " + kl;");
+            w_.line("const unsigned long m0 = 0ul;");
+            w_.line("const unsigned long k0 = tile * " + ul(c_.Kb) + ";");
         } else {
-            w_.line("const unsigned long m = (tile % " + ul(MG_) + ") * " + ul(c_.Mb) + " + ml;");
-            w_.line("const unsigned long kc = (tile / " + ul(MG_) + ") * " + ul(c_.Kb) +
-                    " + kl;");
+            w_.line("const unsigned long m0 = (tile % " + ul(MG_) + ") * " + ul(c_.Mb) + ";");
+            w_.line("const unsigned long k0 = (tile / " + ul(MG_) + ") * " + ul(c_.Kb) + ";");
         }
+        w_.line("const unsigned long m = m0 + ml;");
+        w_.line("const unsigned long kc = k0 + kl;");
         std::string act = "kc < kcols";
         if (MG_ * c_.Mb != c_.M) {
             act = "m < " + ul(c_.M) + " && " + act;
@@ -571,13 +602,25 @@ class kernel_gen {
             act = band_ + " && " + act;
         }
         w_.line("const bool active = " + act + ";");
+        std::string const kf = pairs_ ? "2ul * " : "";
         if (pairs_) {
             w_.line("const bool active_b = active && 2ul * kc + 1ul < K;");
-            w_.line("const unsigned long ib = m + 2ul * kc * " + ul(c_.istride[2]) + ";");
-            w_.line("const unsigned long ob = m + 2ul * kc * " + ul(c_.ostride[2]) + ";");
+        }
+        if (local32_) {
+            // Uniform tile base pointers and 32-bit offsets within the tile: the per-lane
+            // address arithmetic stays 32-bit and the base can live in scalar registers.
+            w_.line("const " + in_type() + "* tin = in + m0 + " + kf + "k0 * " +
+                    ul(c_.istride[2]) + ";");
+            w_.line(out_type() + "* tout = out + m0 + " + kf + "k0 * " + ul(c_.ostride[2]) +
+                    ";");
+            std::string const kl2 = pairs_ ? "2u * kl" : "kl";
+            w_.line("const unsigned ib = ml + " + kl2 + " * " + u(c_.istride[2]) + ";");
+            w_.line("const unsigned ob = ml + " + kl2 + " * " + u(c_.ostride[2]) + ";");
         } else {
-            w_.line("const unsigned long ib = m + kc * " + ul(c_.istride[2]) + ";");
-            w_.line("const unsigned long ob = m + kc * " + ul(c_.ostride[2]) + ";");
+            w_.line("const " + in_type() + "* tin = in;");
+            w_.line(out_type() + "* tout = out;");
+            w_.line("const unsigned long ib = m + " + kf + "kc * " + ul(c_.istride[2]) + ";");
+            w_.line("const unsigned long ob = m + " + kf + "kc * " + ul(c_.ostride[2]) + ";");
         }
 
         if (c_.type == transform_type::c2r) {
@@ -640,13 +683,6 @@ class kernel_gen {
                         " + cc * 32u + hh * 8u));");
             }
         }
-        if (MG_ == 1) {
-            w_.line("const unsigned long m0 = 0ul;");
-            w_.line("const unsigned long k0 = tile * " + ul(c_.Kb) + ";");
-        } else {
-            w_.line("const unsigned long m0 = (tile % " + ul(MG_) + ") * " + ul(c_.Mb) + ";");
-            w_.line("const unsigned long k0 = (tile / " + ul(MG_) + ") * " + ul(c_.Kb) + ";");
-        }
         // All loads of the work-item are issued before the matrix instructions (one memory
         // latency per tile instead of one per 32 columns).
         std::string ld_cond = "cv && k0 + kk < kcols";
@@ -671,16 +707,24 @@ class kernel_gen {
                 }
             }
             w_.line("const bool ld" + x + " = " + ldx + ";");
-            w_.line("const unsigned long bb" + x + " = m0 + mm" + x + " + (k0 + kk" + x + ") * " +
-                    ul(c_.istride[2]) + ";");
+            if (local32_) {
+                w_.line("const unsigned bb" + x + " = mm" + x + " + kk" + x + " * " +
+                        u(c_.istride[2]) + ";");
+            } else {
+                w_.line("const unsigned long bb" + x + " = m0 + mm" + x + " + (k0 + kk" + x +
+                        ") * " + ul(c_.istride[2]) + ";");
+            }
             for (unsigned h2 = 0; h2 < 2; ++h2) {
                 std::string r = "r" + std::to_string(h2) + x;
                 w_.line("bbfft_u4 " + r + " = {0u, 0u, 0u, 0u};");
                 w_.open("if (ld" + x + ")");
                 for (unsigned e = 0; e < 4; ++e) {
                     std::string n1 = "(" + u(8 * h2 + e) + " + 4u * hh)";
-                    std::string off = "bb" + x + " + (unsigned long)(jj" + x + " + " + u(N2) +
-                                      " * " + n1 + ") * " + ul(c_.istride[1]);
+                    std::string off =
+                        local32_ ? "bb" + x + " + (jj" + x + " + " + u(N2) + " * " + n1 + ") * " +
+                                       u(c_.istride[1])
+                                 : "bb" + x + " + (unsigned long)(jj" + x + " + " + u(N2) +
+                                       " * " + n1 + ") * " + ul(c_.istride[1]);
                     w_.line(r + "[" + std::to_string(e) + "] = " + gload(off) + ";");
                 }
                 w_.close();
@@ -714,10 +758,18 @@ class kernel_gen {
 
     /// Offset (elements) of FFT index n in the input column along mode 1.
     std::string ioff(std::string const &n, std::size_t extra_k = 0) const {
+        if (local32_) {
+            std::string k = extra_k ? " + " + u(c_.istride[2] * extra_k) : "";
+            return "ib" + k + " + (unsigned)(" + n + ") * " + u(c_.istride[1]);
+        }
         std::string k = extra_k ? " + " + ul(c_.istride[2] * extra_k) : "";
         return "ib" + k + " + (unsigned long)(" + n + ") * " + ul(c_.istride[1]);
     }
     std::string ooff(std::string const &n, std::size_t extra_k = 0) const {
+        if (local32_) {
+            std::string k = extra_k ? " + " + u(c_.ostride[2] * extra_k) : "";
+            return "ob" + k + " + (unsigned)(" + n + ") * " + u(c_.ostride[1]);
+        }
         std::string k = extra_k ? " + " + ul(c_.ostride[2] * extra_k) : "";
         return "ob" + k + " + (unsigned long)(" + n + ") * " + ul(c_.ostride[1]);
     }
@@ -742,7 +794,7 @@ class kernel_gen {
             }
         } else if (half_) {
             if (vector_real_in()) {
-                std::string p = "((const " + T2() + "*)(in + ib)) + (" + n + ")";
+                std::string p = "((const " + T2() + "*)(tin + ib)) + (" + n + ")";
                 w_.line(v + " = " + ((c_.flags & fft1d_nontemporal_load) ? "ld_nt(" + p + ")"
                                                                           : "*(" + p + ")") +
                         ";");
@@ -762,7 +814,7 @@ class kernel_gen {
             w_.line(gstore(ooff(n), y));
         } else if (half_) {
             if (vector_real_out()) {
-                std::string p = "((" + T2() + "*)(out + ob)) + (" + n + ")";
+                std::string p = "((" + T2() + "*)(tout + ob)) + (" + n + ")";
                 w_.line((c_.flags & fft1d_nontemporal_store) ? "st_nt(" + p + ", " + y + ");"
                                                              : "*(" + p + ") = " + y + ";");
             } else {
@@ -815,10 +867,21 @@ class kernel_gen {
             if (!cond.empty()) {
                 w_.open("if (" + cond + ")");
             }
+            // Reads at j + i NR: with NR a multiple of the padding interval, the padded index
+            // is padded(j) + i (NR + NR / pad), one base per butterfly.
+            std::size_t const padv = c_.lds_pad;
+            bool const rd_split = !src_global && (padv == 0 || NR % padv == 0);
+            std::string const rb = "pr" + std::to_string(s) + "_" + std::to_string(b);
+            if (rd_split) {
+                w_.line("const unsigned " + rb + " = " + padded(jn[b]) + ";");
+            }
             for (std::size_t i = 0; i < R; ++i) {
                 std::string n = jn[b] + " + " + u(i * NR);
                 if (src_global) {
                     emit_global_input(v[b][i], n);
+                } else if (rd_split) {
+                    w_.line(v[b][i] + " = " +
+                            lds_slot(rb, i * (NR + (padv ? NR / padv : 0))) + ";");
                 } else {
                     w_.line(v[b][i] + " = " + lds(n) + ";");
                 }
@@ -870,8 +933,43 @@ class kernel_gen {
                 std::string base = p == 1 ? j + " * " + u(R)
                                           : "(" + j + " / " + u(p) + ") * " + u(p * R) + " + " +
                                                 j + " % " + u(p);
+                // Writes at base + i p: split into padded(base) + a constant when no padding
+                // carry depends on the work-item (p a multiple of the interval, or p = 1 with
+                // R a multiple or a power-of-two divisor of it).
+                std::size_t const padv = c_.lds_pad;
+                auto is_pow2 = [](std::size_t v) { return v && !(v & (v - 1)); };
+                int mode = -1; // -1: no split
+                if (padv == 0 || p % padv == 0) {
+                    mode = 0;
+                } else if (p == 1 && R % padv == 0) {
+                    mode = 1;
+                } else if (p == 1 && is_pow2(R) && padv % R == 0) {
+                    mode = 2;
+                }
+                std::string const wb = "pw" + std::to_string(s) + "_" + std::to_string(b);
+                if (mode >= 0) {
+                    w_.line("const unsigned " + wb + " = " + padded(base) + ";");
+                }
                 for (std::size_t i = 0; i < R; ++i) {
-                    w_.line(lds(base + " + " + u(i * p)) + " = " + y[i] + ";");
+                    std::size_t off = 0;
+                    switch (mode) {
+                    case 0:
+                        off = i * p + (padv ? i * p / padv : 0);
+                        break;
+                    case 1:
+                        off = i + i / padv;
+                        break;
+                    case 2:
+                        off = i;
+                        break;
+                    default:
+                        break;
+                    }
+                    if (mode >= 0) {
+                        w_.line(lds_slot(wb, off) + " = " + y[i] + ";");
+                    } else {
+                        w_.line(lds(base + " + " + u(i * p)) + " = " + y[i] + ";");
+                    }
                 }
             }
             w_.close();
@@ -1034,6 +1132,7 @@ class kernel_gen {
     unsigned pad_shift_ = 0;
     bool real_ = false, half_ = false, pairs_ = false, use_lds_ = false;
     bool half_in_ = false, bf16_ = false, mfma_ = false;
+    bool local32_ = false;
 };
 
 } // namespace
