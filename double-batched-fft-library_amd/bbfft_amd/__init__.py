@@ -214,6 +214,7 @@ def tuning_struct(t):
         setattr(c, k, int(t.get(k, 0)))
     c.no_pad = int(bool(t.get("no_pad", False)))
     c.waves_per_eu = int(t.get("waves_per_eu", 0))
+    c.groups_per_cu = int(t.get("groups_per_cu", 0))
     if "flags" in t:
         c.flags = int(t["flags"])
         c.has_flags = 1

@@ -75,6 +75,7 @@ class CTuning(ctypes.Structure):
         ("Nt", ctypes.c_size_t), ("Mb", ctypes.c_size_t), ("Kb", ctypes.c_size_t),
         ("lds_pad", ctypes.c_size_t), ("no_pad", ctypes.c_int), ("flags", ctypes.c_uint),
         ("has_flags", ctypes.c_int), ("waves_per_eu", ctypes.c_uint),
+        ("groups_per_cu", ctypes.c_uint),
     ]
 
 

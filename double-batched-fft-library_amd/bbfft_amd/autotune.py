@@ -24,7 +24,8 @@ from . import (GFX950_INFO, BadConfiguration, BbfftError, default_cache, kernel_
 
 TABLE_HEADER = ("# Tuned kernel parameters for gfx950 (MI355X), written by "
                 "`python -m bbfft_amd.autotune`.\n"
-                "# precision type N M : factorization Nt Mb Kb lds_pad flags [waves_per_eu]"
+                "# precision type N M : factorization Nt Mb Kb lds_pad flags "
+                "[waves_per_eu [groups_per_cu]]"
                 "   # GB/s default -> tuned, descriptor\n")
 
 
@@ -170,7 +171,8 @@ class Tuner:
         kc = kernel_configuration(self.cfg, tuning=partial, device_info=GFX950_INFO)
         return {"factorization": kc["factorization"], "Nt": kc["Nt"], "Mb": kc["Mb"],
                 "Kb": kc["Kb"], "lds_pad": kc["lds_pad"], "no_pad": kc["lds_pad"] == 0,
-                "flags": partial["flags"], "waves_per_eu": partial.get("waves_per_eu", 0)}
+                "flags": partial["flags"], "waves_per_eu": partial.get("waves_per_eu", 0),
+                "groups_per_cu": partial.get("groups_per_cu", 0)}
 
     def factorizations(self):
         if self.variant & 32:  # the MFMA stage is a first radix of 16
@@ -218,6 +220,9 @@ class Tuner:
             # 6) occupancy target of the register allocator
             opts = [dict(best, waves_per_eu=w) for w in (0, 4, 5, 6, 8)]
             best, best_ms = self._try(best, best_ms, opts)
+            # 7) persistent grid (work-groups per CU looping over the tiles)
+            opts = [dict(best, groups_per_cu=g) for g in (0, 2, 4, 8, 16)]
+            best, best_ms = self._try(best, best_ms, opts)
         return best, best_ms
 
     def _try(self, best, best_ms, options, complete=False):
@@ -242,8 +247,10 @@ def table_line(cfg, tuning, default_ms, best_ms, nbytes, desc):
         typ += "-mfma"
     fac = "x".join(str(f) for f in tuning["factorization"])
     waves = tuning.get("waves_per_eu", 0)
+    groups = tuning.get("groups_per_cu", 0)
+    extra = f" {waves} {groups}" if groups else (f" {waves}" if waves else "")
     return (f"{prec} {typ} {cfg.shape[1]} {cfg.shape[0]} : {fac} {tuning['Nt']} {tuning['Mb']} "
-            f"{tuning['Kb']} {tuning['lds_pad']} {flags & 3}{f' {waves}' if waves else ''}   # "
+            f"{tuning['Kb']} {tuning['lds_pad']} {flags & 3}{extra}   # "
             f"{nbytes / default_ms / 1e6:.0f} -> {nbytes / best_ms / 1e6:.0f} GB/s, {desc}")
 
 

@@ -151,6 +151,7 @@ fft1d_tuning tuning_of(bbfft_tuning const *t) {
     r.flags = t->flags;
     r.has_flags = t->has_flags != 0;
     r.waves_per_eu = t->waves_per_eu;
+    r.groups_per_cu = t->groups_per_cu;
     return r;
 }
 

@@ -262,6 +262,7 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
         throw bad_configuration("waves_per_eu must be 0..8");
     }
     c.waves_per_eu = tuning.waves_per_eu;
+    c.groups_per_cu = tuning.groups_per_cu;
     if ((c.flags & fft1d_adjoint) && cfg.type == transform_type::c2c) {
         throw bad_configuration("The adjoint variant applies to real transforms only");
     }

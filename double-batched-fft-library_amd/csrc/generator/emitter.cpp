@@ -744,10 +744,22 @@ class kernel_gen {
             w_.open("if (cv" + x + ")");
             w_.line(T2() + "* cp = lds_buf + mm" + x + " + " + u(c_.Mb * np_) + " * kk" + x +
                     ";");
+            // slot jj * 16 + k1 (k1 < 16): with no padding or a padding interval of at least
+            // 16, the padded index is padded(jj * 16) + k1
+            bool const split = c_.lds_pad == 0 || c_.lds_pad >= 16;
+            if (split) {
+                w_.line("const unsigned pj = " + padded("jj" + x + " * 16u") + " + 2u * hh;");
+            }
             for (unsigned r = 0; r < 16; r += 2) {
-                std::string k1 = "(" + u(4 * (r / 4) + (r % 4) / 2) + " + 2u * hh)";
-                w_.line(lds_at("cp", "jj" + x + " * 16u + " + k1) + " = mk(acc[" +
-                        std::to_string(r) + "], acc[" + std::to_string(r + 1) + "]);");
+                std::size_t const kc = 4 * (r / 4) + (r % 4) / 2;
+                std::string const val = "mk(acc[" + std::to_string(r) + "], acc[" +
+                                        std::to_string(r + 1) + "])";
+                if (split) {
+                    w_.line("cp[(pj + " + u(kc) + ") * " + u(c_.Mb) + "] = " + val + ";");
+                } else {
+                    std::string k1 = "(" + u(kc) + " + 2u * hh)";
+                    w_.line(lds_at("cp", "jj" + x + " * 16u + " + k1) + " = " + val + ";");
+                }
             }
             w_.close();
             w_.close();

@@ -67,9 +67,12 @@ std::map<key_t, fft1d_tuning> parse(std::string const &text) {
         while (std::getline(fs, f, 'x')) {
             t.factorization.push_back(std::stoi(f));
         }
-        unsigned waves = 0;
+        unsigned waves = 0, groups = 0;
         if (rhs >> waves) { // optional occupancy target
             t.waves_per_eu = waves;
+            if (rhs >> groups) { // optional persistent grid
+                t.groups_per_cu = groups;
+            }
         }
         t.no_pad = t.lds_pad == 0;
         t.has_flags = true;

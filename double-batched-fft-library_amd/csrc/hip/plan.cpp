@@ -56,6 +56,13 @@ fft1d_plan::fft1d_plan(configuration const &cfg, hipStream_t stream, int device,
     std::size_t const kcols = kc_.K_columns(K_);
     std::size_t const ntiles = mg * ((kcols + kc_.Kb - 1) / kc_.Kb);
     grid_ = static_cast<unsigned>(std::min<std::size_t>(ntiles, std::size_t(1) << 24));
+    if (kc_.groups_per_cu) {
+        int cus = 0;
+        BBFFT_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        grid_ = static_cast<unsigned>(std::max<std::size_t>(
+            1, std::min<std::size_t>(ntiles, std::size_t(kc_.groups_per_cu) *
+                                                 static_cast<std::size_t>(std::max(cus, 1)))));
+    }
     block_ = static_cast<unsigned>(kc_.work_group_size());
     std::size_t const rb = static_cast<std::size_t>(kc_.fp);
     // Real data read or written as (x[2n], x[2n+1]) pairs needs complex alignment.

@@ -44,6 +44,7 @@ struct BBFFT_EXPORT fft1d_configuration {
     std::size_t lds_pad;                 ///< one padding slot per lds_pad slots (0 = none)
     unsigned flags;                      ///< fft1d_flag bits (global access hints)
     unsigned waves_per_eu;               ///< occupancy target of the compiler (0 = default)
+    unsigned groups_per_cu;              ///< persistent grid: work-groups per CU (0 = a tile each)
     bool inplace_unsupported;            ///< in == out would race across work-groups
     char const *load_function;           ///< user load callback or nullptr
     char const *store_function;          ///< user store callback or nullptr
@@ -100,6 +101,9 @@ struct BBFFT_EXPORT fft1d_tuning {
     /// Waves per SIMD the register allocator must allow (0 = its default); a higher target
     /// trades registers (possibly spills) for occupancy.
     unsigned waves_per_eu = 0;
+    /// Persistent launch: this many work-groups per CU loop over the tiles (0 = one work-group
+    /// per tile). Does not change the kernel, only its grid.
+    unsigned groups_per_cu = 0;
 };
 
 /// Choose a kernel configuration for a 1-D configuration (dim == 1). `variant` (fft1d_flag

@@ -77,6 +77,7 @@ typedef struct bbfft_tuning {
     unsigned flags;
     int has_flags;
     unsigned waves_per_eu; /* occupancy target of the compiler, 0..8 (0 = its default) */
+    unsigned groups_per_cu; /* persistent grid: work-groups per CU (0 = one per tile) */
 } bbfft_tuning;
 
 /* Message of the last failed call on this thread ("" if none). */
