@@ -868,7 +868,7 @@ class kernel_gen {
             for (std::size_t i = 0; i < R; ++i) {
                 v[b][i] = "v" + std::to_string(s) + "_" + std::to_string(b) + "_" +
                           std::to_string(i);
-                w_.line(T2() + " " + v[b][i] + " = " + f_.zero2() + ";");
+                w_.line(T2() + " " + v[b][i] + ";");
             }
             std::string cond = guard[b];
             if (src_global) {
@@ -906,6 +906,13 @@ class kernel_gen {
                 }
             }
             if (!cond.empty()) {
+                // Work-items without a butterfly (or without a column) hold zeros; the branch
+                // is skipped entirely when a whole wave is active.
+                w_.close();
+                w_.open("else");
+                for (std::size_t i = 0; i < R; ++i) {
+                    w_.line(v[b][i] + " = " + f_.zero2() + ";");
+                }
                 w_.close();
             }
         }
