@@ -32,3 +32,19 @@ def test_benchmark_cli_rows(args):
     for row in lines[1:]:
         f = row.split(",")
         assert f[0] in "sd" and f[5] in "rc" and float(f[7]) > 0 and float(f[8]) > 0
+
+
+@pytest.mark.parametrize("args", [
+    ["-d", "-k", "40", "16.16", "70.16", "128.512"],
+    ["-s", "-k", "25", "1120.32", "3.60"],
+])
+def test_transpose_fft_transpose(args):
+    """Transpose + unit-stride plan + transpose equals the M-batched plan (reference
+    examples/transpose_fft_transpose/tft.cpp:148-168 checks the same equality)."""
+    from bbfft_amd import tft
+    out = io.StringIO()
+    with contextlib.redirect_stdout(out):
+        assert tft.main(args + ["--reps", "2"]) == 0
+    text = out.getvalue()
+    assert text.count("Speed-up:") == len(args) - 2
+    assert "Error" not in text
