@@ -816,7 +816,7 @@ class kernel_gen {
             }
         } else { // pairs
             w_.line(v + ".x = " + gload(ioff(n)) + ";");
-            w_.line("if (active_b) " + v + ".y = " + gload(ioff(n, 1)) + ";");
+            w_.line(v + ".y = active_b ? " + gload(ioff(n, 1)) + " : " + f_.lit(0.0) + ";");
         }
     }
 

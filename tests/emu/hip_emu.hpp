@@ -12,11 +12,13 @@
 #include <ucontext.h>
 #include <vector>
 
+// Vector values start as NaN so that a kernel reading a component it never wrote (on the GPU:
+// whatever the register held) produces NaN in the emulated output instead of a benign zero.
 struct float2 {
-    float x, y;
+    float x = __builtin_nanf(""), y = __builtin_nanf("");
 };
 struct double2 {
-    double x, y;
+    double x = __builtin_nan(""), y = __builtin_nan("");
 };
 struct emu_dim3 {
     unsigned x = 0, y = 0, z = 0;

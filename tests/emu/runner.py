@@ -17,7 +17,9 @@ CLANG = "/opt/rocm/llvm/bin/clang++"
 
 def _build_so(tu):
     os.makedirs(_BUILD, exist_ok=True)
-    h = hashlib.sha1(tu.encode()).hexdigest()[:16]
+    with open(os.path.join(_HERE, "hip_emu.hpp"), "rb") as f:
+        shim = f.read()  # the shim is part of every translation unit's cache key
+    h = hashlib.sha1(tu.encode() + shim).hexdigest()[:16]
     so = os.path.join(_BUILD, f"k_{h}.so")
     if not os.path.exists(so):
         src = os.path.join(_BUILD, f"k_{h}.cpp")

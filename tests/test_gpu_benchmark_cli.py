@@ -46,5 +46,5 @@ def test_transpose_fft_transpose(args):
     with contextlib.redirect_stdout(out):
         assert tft.main(args + ["--reps", "2"]) == 0
     text = out.getvalue()
-    assert text.count("Speed-up:") == len(args) - 2
+    assert text.count("Speed-up:") == len(args) - 3
     assert "Error" not in text

@@ -54,7 +54,7 @@ def test_half_input_launch_and_alignment():
     cfg = _cfg(4, 512, 64)
     plan = bb.make_half_input_plan(cfg, "f16", True, cache=bb.default_cache())
     (launch,) = plan.launches()
-    kc = bb.kernel_configuration(cfg, tuning={"flags": bb.half_input_flags("f16", True)})
+    kc = bb.half_input_kernel_configuration(cfg, "f16", True)  # tuning table applied
     assert launch["kernel"] == kc["name"] and launch["block"] % 64 == 0
     x = torch.zeros(4 * 512 * 64 + 1, dtype=torch.int32, device="cuda")
     y = torch.zeros(4 * 512 * 64, dtype=torch.complex64, device="cuda")
