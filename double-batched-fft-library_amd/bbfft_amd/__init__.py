@@ -441,6 +441,7 @@ def generate_fused_source(cfg: configuration, device_info=GFX950_INFO) -> str:
 INPUT_F16 = _capi.INPUT_F16
 INPUT_BF16 = _capi.INPUT_BF16
 MFMA = _capi.MFMA
+PREFETCH = _capi.PREFETCH  # software-pipelined tile loop (fft1d flag)
 
 
 def half_input_flags(storage="f16", mfma=True) -> int:

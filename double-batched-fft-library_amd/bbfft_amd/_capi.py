@@ -107,6 +107,7 @@ RAGGED_GRAPH = 1
 INPUT_F16 = 8
 INPUT_BF16 = 16
 MFMA = 32
+PREFETCH = 64
 
 _lib = None
 

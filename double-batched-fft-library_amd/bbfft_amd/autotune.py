@@ -19,8 +19,8 @@ import os
 import sys
 import time
 
-from . import (GFX950_INFO, BadConfiguration, BbfftError, default_cache, kernel_configuration,
-               make_plan, parse_fft_descriptor)
+from . import (GFX950_INFO, PREFETCH, BadConfiguration, BbfftError, default_cache,
+               kernel_configuration, make_plan, parse_fft_descriptor)
 
 TABLE_HEADER = ("# Tuned kernel parameters for gfx950 (MI355X), written by "
                 "`python -m bbfft_amd.autotune`.\n"
@@ -215,13 +215,17 @@ class Tuner:
             opts = [dict(best, lds_pad=p, no_pad=p == 0) for p in (0, 8, 16, 32)]
             best, best_ms = self._try(best, best_ms, opts)
             # 5) streaming hints
-            opts = [dict(best, flags=fl | self.variant) for fl in (0, 1, 2, 3)]
+            opts = [dict(best, flags=fl | (best["flags"] & PREFETCH) | self.variant)
+                    for fl in (0, 1, 2, 3)]
             best, best_ms = self._try(best, best_ms, opts)
             # 6) occupancy target of the register allocator
             opts = [dict(best, waves_per_eu=w) for w in (0, 4, 5, 6, 8)]
             best, best_ms = self._try(best, best_ms, opts)
-            # 7) persistent grid (work-groups per CU looping over the tiles)
-            opts = [dict(best, groups_per_cu=g) for g in (0, 2, 4, 8, 16)]
+            # 7) persistent grid (work-groups per CU looping over the tiles), with and without
+            #    loading the next tile's inputs during the current tile
+            pfs = (0, PREFETCH) if self.type != 2 and not self.variant & 32 else (0,)
+            opts = [dict(best, groups_per_cu=g, flags=(best["flags"] & ~PREFETCH) | pf)
+                    for g in (0, 1, 2, 3, 4, 6, 8, 16) for pf in pfs]
             best, best_ms = self._try(best, best_ms, opts)
         return best, best_ms
 

@@ -263,6 +263,12 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     }
     c.waves_per_eu = tuning.waves_per_eu;
     c.groups_per_cu = tuning.groups_per_cu;
+    if (c.flags & fft1d_prefetch) {
+        if (cfg.type == transform_type::c2r || cfg.callbacks || (c.flags & fft1d_mfma)) {
+            throw bad_configuration(
+                "Tile prefetching requires a c2c or r2c kernel without callbacks or MFMA stage");
+        }
+    }
     if ((c.flags & fft1d_adjoint) && cfg.type == transform_type::c2c) {
         throw bad_configuration("The adjoint variant applies to real transforms only");
     }

@@ -347,6 +347,8 @@ class kernel_gen {
         half_in_ = (c.flags & (fft1d_input_f16 | fft1d_input_bf16)) != 0;
         bf16_ = (c.flags & fft1d_input_bf16) != 0;
         mfma_ = (c.flags & fft1d_mfma) != 0;
+        prefetch_ = (c.flags & fft1d_prefetch) != 0 && local32_ && !mfma_ &&
+                    c.type != transform_type::c2r;
     }
 
     std::string generate() {
@@ -362,6 +364,7 @@ class kernel_gen {
     /// persistent ragged kernel. Work-items beyond Mb * Nt * Kb of a `group_size` work-group
     /// take part in the barriers only.
     std::string generate_body() {
+        prefetch_ = false; // one tile per call: nothing to pipeline
         emit_tables();
         emit_body();
         return w_.str();
@@ -546,6 +549,20 @@ class kernel_gen {
         if (use_lds_) {
             w_.line(T2() + "* col = lds_buf + ml + " + u(c_.Mb * np_) + " * kl;");
         }
+        if (prefetch_) {
+            // Stage-0 inputs of the work-group's first tile; each iteration then loads those
+            // of the next tile while it transforms the current one.
+            for (auto const &q : prefetch_regs()) {
+                for (auto const &qi : q) {
+                    w_.line(T2() + " " + qi + ";");
+                }
+            }
+            w_.open("if (blockIdx.x < ntiles)");
+            w_.line("const unsigned long tile = blockIdx.x;");
+            emit_tile_origin();
+            emit_stage0_loads();
+            w_.close();
+        }
         w_.open("for (unsigned long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)");
         emit_tile();
         if (use_lds_) {
@@ -584,6 +601,65 @@ class kernel_gen {
 
     /// One (M tile, K tile) of work: `tile` selects it.
     void emit_tile() {
+        emit_tile_origin();
+        if (c_.type == transform_type::c2r) {
+            emit_c2r_pre();
+            w_.line("__syncthreads();");
+        }
+        for (std::size_t s = 0; s < L_; ++s) {
+            if (s == 0 && mfma_) {
+                emit_mfma_stage0();
+            } else {
+                emit_stage(s);
+            }
+        }
+        if (c_.type == transform_type::r2c) {
+            emit_r2c_post();
+        }
+    }
+
+    /// Register names q<b>_<i> holding the prefetched stage-0 inputs (butterfly b, input i).
+    std::vector<std::vector<std::string>> prefetch_regs() const {
+        std::size_t const R = c_.factorization[0];
+        std::size_t const NR = nfft_ / R;
+        std::size_t const B = (NR + c_.Nt - 1) / c_.Nt;
+        std::vector<std::vector<std::string>> q(B, std::vector<std::string>(R));
+        for (std::size_t b = 0; b < B; ++b) {
+            for (std::size_t i = 0; i < R; ++i) {
+                q[b][i] = "q" + std::to_string(b) + "_" + std::to_string(i);
+            }
+        }
+        return q;
+    }
+
+    /// Loads the stage-0 inputs of the tile whose origin is in scope into the prefetch
+    /// registers (zeros for work-items without a butterfly or column).
+    void emit_stage0_loads() {
+        std::size_t const R = c_.factorization[0];
+        std::size_t const NR = nfft_ / R;
+        auto const q = prefetch_regs();
+        for (std::size_t b = 0; b < q.size(); ++b) {
+            std::string const j = "j0_" + std::to_string(b);
+            w_.line("const unsigned " + j + " = t + " + u(b * c_.Nt) + ";");
+            std::string cond = "active";
+            if ((b + 1) * c_.Nt > NR) {
+                cond += " && " + j + " < " + u(NR);
+            }
+            w_.open("if (" + cond + ")");
+            for (std::size_t i = 0; i < R; ++i) {
+                emit_global_input(q[b][i], j + " + " + u(i * NR));
+            }
+            w_.close();
+            w_.open("else");
+            for (std::size_t i = 0; i < R; ++i) {
+                w_.line(q[b][i] + " = " + f_.zero2() + ";");
+            }
+            w_.close();
+        }
+    }
+
+    /// Origin, activity and base pointers of the tile `tile` (in scope).
+    void emit_tile_origin() {
         // tile origin (uniform across the work-group)
         if (MG_ == 1) {
             w_.line("const unsigned long m0 = 0ul;");
@@ -621,21 +697,6 @@ class kernel_gen {
             w_.line(out_type() + "* tout = out;");
             w_.line("const unsigned long ib = m + " + kf + "kc * " + ul(c_.istride[2]) + ";");
             w_.line("const unsigned long ob = m + " + kf + "kc * " + ul(c_.ostride[2]) + ";");
-        }
-
-        if (c_.type == transform_type::c2r) {
-            emit_c2r_pre();
-            w_.line("__syncthreads();");
-        }
-        for (std::size_t s = 0; s < L_; ++s) {
-            if (s == 0 && mfma_) {
-                emit_mfma_stage0();
-            } else {
-                emit_stage(s);
-            }
-        }
-        if (c_.type == transform_type::r2c) {
-            emit_r2c_post();
         }
     }
 
@@ -870,6 +931,13 @@ class kernel_gen {
                           std::to_string(i);
                 w_.line(T2() + " " + v[b][i] + ";");
             }
+            if (src_global && prefetch_) {
+                auto const q = prefetch_regs();
+                for (std::size_t i = 0; i < R; ++i) {
+                    w_.line(v[b][i] + " = " + q[b][i] + ";");
+                }
+                continue;
+            }
             std::string cond = guard[b];
             if (src_global) {
                 cond = cond.empty() ? "active" : "active && " + cond;
@@ -915,6 +983,17 @@ class kernel_gen {
                 }
                 w_.close();
             }
+        }
+        if (src_global && prefetch_) {
+            // the next tile's inputs are in flight while this tile is transformed
+            w_.open("");
+            w_.line("const unsigned long tile_next = tile + gridDim.x;");
+            w_.open("if (tile_next < ntiles)");
+            w_.line("const unsigned long tile = tile_next;");
+            emit_tile_origin();
+            emit_stage0_loads();
+            w_.close();
+            w_.close();
         }
         if (mid_barrier) {
             w_.line("__syncthreads();");
@@ -1152,6 +1231,7 @@ class kernel_gen {
     bool real_ = false, half_ = false, pairs_ = false, use_lds_ = false;
     bool half_in_ = false, bf16_ = false, mfma_ = false;
     bool local32_ = false;
+    bool prefetch_ = false;
 };
 
 } // namespace

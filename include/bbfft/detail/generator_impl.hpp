@@ -83,6 +83,12 @@ enum fft1d_flag : unsigned {
     /// DFT matrix is split into 2 (f16) or 3 (bf16) half-precision terms whose products with the
     /// exactly representable input are accumulated, so the stage is accurate to fp32 level.
     fft1d_mfma = 32u,
+    /// Software-pipelined tile loop: while a work-group transforms one tile it already loads the
+    /// first-stage inputs of its next tile (tile + number of work-groups) into registers, so
+    /// the memory latency of a tile overlaps the LDS stages and stores of the previous one.
+    /// Pays off with a persistent grid (groups_per_cu > 0). c2c and r2c without callbacks;
+    /// not with fft1d_mfma.
+    fft1d_prefetch = 64u,
 };
 
 /// Number of half-precision terms the DFT matrix is split into on the MFMA stage.

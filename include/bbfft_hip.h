@@ -67,8 +67,11 @@ typedef struct bbfft_kernel_config {
 } bbfft_kernel_config;
 
 /* Overrides of the tiling heuristics of a 1-D plan (0 / num_factors == 0: automatic).
- * flags: bit 0 non-temporal global loads, bit 1 non-temporal global stores; used only when
- * has_flags != 0. (No reference counterpart: the reference's tiling is fixed by device_info.) */
+ * flags: bit 0 non-temporal global loads, bit 1 non-temporal global stores, BBFFT_PREFETCH
+ * (c2c / r2c without callbacks: each work-group loads its next tile's inputs while it transforms
+ * the current one; for persistent grids, groups_per_cu > 0); used only when has_flags != 0.
+ * (No reference counterpart: the reference's tiling is fixed by device_info.) */
+enum { BBFFT_PREFETCH = 64 };
 typedef struct bbfft_tuning {
     int factorization[8];
     int num_factors;

@@ -161,3 +161,40 @@ def test_adjoint_kernels_inner_product_identity(N, t):
     ATy = runner.run(bb, adj, y, n_in, tuning=variant)
     lhs, rhs = _real_inner(Ax, y), _real_inner(x, ATy)
     assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + np.linalg.norm(Ax) * np.linalg.norm(y))
+
+
+@pytest.mark.parametrize("fp", [4, 8])
+@pytest.mark.parametrize("t,N", [(0, 16), (0, 60), (0, 256), (0, 105), (1, 64), (1, 27), (1, 10)])
+@pytest.mark.parametrize("cap", [1, 2, 3])
+def test_prefetch_tile_loop(t, N, fp, cap):
+    """Software-pipelined tile loop (fft1d flag 64): with `cap` work-groups every group walks
+    several tiles, loading the next tile's inputs while it transforms the current one; edge
+    tiles (M not a multiple of Mb, odd K for the paired odd-N real kernels) included."""
+    for M, K, inplace in [(3, 7, False), (5, 4, True), (1, 9, False)]:
+        cfg = _cfg(M, N, K, fp, -1, t, inplace)
+        # in-place real transforms need the whole M in one tile (the reference's rule)
+        tuning = {"flags": bb.PREFETCH, "Mb": M if (t and inplace) else min(M, 2), "Kb": 1}
+        kc = bb.kernel_configuration(cfg, tuning=tuning)
+        assert kc["name"].endswith("_g64")
+        n_in = oracle.extent(1, cfg.shape, cfg.istride)
+        n_out = oracle.extent(1, [M, N // 2 + 1 if t else N, K], cfg.ostride)
+        x = oracle.uniform_real(N, n_in) if t else oracle.uniform_complex(N, n_in)
+        ref = oracle.fft(1, cfg.shape, cfg.istride, cfg.ostride, t, -1, x, out_size=n_out)
+        if inplace:
+            rt = np.float32 if fp == 4 else np.float64
+            ct = np.complex64 if fp == 4 else np.complex128
+            buf = np.zeros(max(n_in, 2 * n_out) if t else 2 * max(n_in, n_out), rt)
+            if t:
+                buf[:n_in] = x
+            else:
+                buf.view(ct)[:n_in] = x
+            y = runner.run(bb, cfg, None, 0, in_place_buffer=buf, tuning=tuning,
+                           grid_cap=cap).view(ct)[:n_out]
+        else:
+            y = runner.run(bb, cfg, x, n_out, grid_cap=cap, tuning=tuning)
+        assert oracle.rel_l2(y, ref) <= oracle.tolerance(fp, N)
+
+
+def test_prefetch_rejected_where_unsupported():
+    with pytest.raises(bb.BadConfiguration):
+        bb.kernel_configuration(_cfg(2, 16, 3, 4, 1, 2), tuning={"flags": bb.PREFETCH})
