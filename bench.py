@@ -66,6 +66,40 @@ def time_plan(bb, torch, desc, steps, warmup, cache):
             "frac": nbytes / ms / 1e6 / HBM_PEAK_GBPS, "bytes": nbytes, "kernel": launch}
 
 
+def stream_copy_bandwidth(torch, nbytes=1 << 30, reps=10):
+    """Read + write GB/s of the hand-written gfx950 copy kernels (lib/libbbfft_stream.so):
+    the practical streaming ceiling for a 1 GiB -> 1 GiB pass. None if the probe is not built."""
+    import ctypes
+    so = os.path.join(ROOT, "double-batched-fft-library_amd", "lib", "libbbfft_stream.so")
+    if not os.path.exists(so):
+        return None
+    fn = ctypes.CDLL(so).bbfft_stream_copy
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                   ctypes.c_void_p]
+    fn.restype = ctypes.c_int
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for kind, name in ((0, "16B_per_item"), (1, "16KiB_tiles")):
+        def run():
+            rc = fn(a.data_ptr(), b.data_ptr(), nbytes, kind, stream)
+            if rc != 0:
+                raise RuntimeError(f"bbfft_stream_copy failed ({rc})")
+        run()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            run()
+        e.record()
+        torch.cuda.synchronize()
+        out[name] = round(2.0 * nbytes * reps / (s.elapsed_time(e) / 1e3) / 1e9, 1)
+    assert torch.equal(a, b)
+    del a, b
+    return out
+
+
 def copy_bandwidth(torch, nbytes=1 << 30, reps=10):
     a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
     b = torch.empty_like(a)
@@ -294,6 +328,7 @@ def main():
         rag = ragged_serving(bb, torch, cache, a.steps)
         print(json.dumps({"ragged": rag}), file=sys.stderr, flush=True)
     copy_gbps = copy_bandwidth(torch)
+    stream = stream_copy_bandwidth(torch)
     cpu = None if a.no_cpu else cpu_baseline()
     cfg = bb.parse_fft_descriptor(a.desc)
     line = {
@@ -312,7 +347,10 @@ def main():
         "config": {"workload": f"c2c forward {a.desc}: M={cfg.shape[0]} x N={cfg.shape[1]} "
                                f"x K={cfg.shape[2]}, out-of-place, uniform[-1,1) complex",
                    "descriptor": a.desc, "HBM_GBps": round(head["GBps"], 1),
-                   "hbm_copy_GBps": round(copy_gbps, 1), "kernel": head["kernel"]["kernel"],
+                   "hbm_copy_GBps": round(copy_gbps, 1), "stream_copy_GBps": stream,
+                   "frac_of_stream_copy": (round(head["GBps"] / max(stream.values()), 4)
+                                           if stream else None),
+                   "kernel": head["kernel"]["kernel"],
                    "sweep": sweep, "nd_real_3d_32x128cubed": nd, "mfma_split_n512": mf,
                    "ragged_serving": rag},
         "roofline": {"bound": "hbm", "achieved": round(head["GBps"], 1), "peak": HBM_PEAK_GBPS,
