@@ -45,7 +45,9 @@ def test_table_entries_are_valid_configurations():
         base, *storage = typ.split("-")
         nfft = int(N) // 2 if base != "c2c" and int(N) % 2 == 0 else int(N)
         assert math.prod(fac) == nfft, ln
-        assert flags in (0, 1, 2, 3), ln  # access hints only; storage bits come from the type
+        # access hints and the load-first tile loop; storage bits come from the type
+        assert flags & ~(3 | bb.PREFETCH) == 0, ln
+        assert not flags & bb.PREFETCH or typ in ("c2c", "r2c"), ln
         if storage:  # half-input kernels: c2c fp32, MFMA entries start with radix 16
             assert base == "c2c" and prec == "f32" and set(storage) <= {"f16", "bf16", "mfma"}
             assert "mfma" not in storage or fac[0] == 16, ln
