@@ -175,8 +175,11 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
         }
         c.factorization = tuning.factorization;
     } else {
-        std::size_t single_max = real_bytes == 4 ? 16 : 16;
-        std::size_t radix_max = real_bytes == 4 ? 32 : 16;
+        // Radices up to 16: a larger radix means more elements per work-item (registers) and
+        // fewer work-items per transform. On MI355X the 2-stage radix-20..32 splits of N = 243,
+        // 384, 500, 512 at M <= 4 reached 3.1-5.8 TB/s against 5.7-6.6 TB/s with radix <= 16.
+        std::size_t const single_max = 16;
+        std::size_t const radix_max = 16;
         c.factorization = gen::default_factorization(nfft, single_max, radix_max);
         if (mfma) {
             c.factorization = gen::default_factorization(nfft / 16, 16, 16);

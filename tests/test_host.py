@@ -232,7 +232,7 @@ def test_unit_stride_m_required():
 @pytest.mark.parametrize("desc,fac,Nt,Mb,Kb", [
     ("scfo3.256*32768", [16, 16], 16, 3, 5),
     ("scfo3.16*32768", [16], 1, 3, 85),
-    ("scfo3.512*32768", [32, 16], 16, 3, 5),
+    ("scfo3.512*32768", [8, 8, 8], 64, 3, 1),
     ("dcfo3.512*32768", [8, 8, 8], 64, 3, 1),
     ("scfo3.60*32768", [10, 6], 6, 3, 14),
     ("scfo2.64*16384", [8, 8], 8, 2, 16),
