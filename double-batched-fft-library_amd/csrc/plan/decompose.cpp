@@ -125,14 +125,15 @@ bool find_fused_pair(pass_layout const &layout, device_info const &info,
     if (!configure_fused2(pa, pb, info, va, vb, fc)) {
         return false;
     }
-    // One work-group per plane, which must overlap its memory latency with other work-groups
-    // on the same CU: planes above 48 KB of LDS (fewer than 3 resident work-groups; e.g. the
-    // 128 x 65 fp32 plane of 128^3 real transforms, measured slower fused than as two passes)
-    // and few large planes are left to the unfused passes, except for problems small enough to
-    // be launch-bound (one launch less). BBFFT_FUSE=on fuses whenever it fits.
+    // One work-group per plane. Planes of up to 96 KB of LDS fuse: the 70 KB plane of the
+    // 32 x 128^3 fp32 real transforms (one 1024-work-item group per CU) measured 162 us fused
+    // against 98 + 91 us for the two passes (r2c) and 169 against 100 + 115 us (c2r) on MI355X.
+    // Larger planes (unmeasured) and few planes (under 256: too few groups for the device) are
+    // left to the unfused passes, except for problems small enough to be launch-bound (one
+    // launch less). BBFFT_FUSE=on fuses whenever it fits.
     std::size_t const total = fc.planes * fc.plane_elems;
     bool const small = total <= (std::size_t(1) << 20);
-    if (!forced && !small && (fc.lds_bytes() > 48 * 1024 || fc.planes < 256)) {
+    if (!forced && !small && (fc.lds_bytes() > 96 * 1024 || fc.planes < 256)) {
         return false;
     }
     out = fused_pair{first, fc};

@@ -81,12 +81,16 @@ def test_fused_in_place_layout_r2c():
 
 
 def test_fusion_policy(monkeypatch):
-    """Planes of at most 48 KB of LDS fuse (several work-groups per CU); the 128 x 65 fp32
-    plane of the 32 x 128^3 real transforms (70 KB) is left to the unfused passes unless
-    BBFFT_FUSE=on; planes that do not fit never fuse; small problems fuse (launch-bound)."""
+    """Planes of at most 96 KB of LDS fuse, among them the 128 x 65 fp32 plane of the
+    32 x 128^3 real transforms (70 KB); larger planes (the 138 KB fp32 128 x 128 plane) are left
+    to the unfused passes unless BBFFT_FUSE=on; planes that do not fit never fuse; small
+    problems fuse (launch-bound)."""
     c3 = bb.parse_fft_descriptor("srfo128x128x128*32")
     c2r = _cfg(1, [128, 128, 128], 32, 4, 2)
-    assert bb.fused_configuration(c3) is None
+    assert bb.fused_configuration(c3) is not None
+    assert bb.fused_configuration(c2r) is not None
+    big = bb.parse_fft_descriptor("scfo128x128*512")
+    assert bb.fused_configuration(big) is None
     assert bb.fused_configuration(bb.parse_fft_descriptor("scfo64x64x64*16")) is not None
     assert bb.fused_configuration(bb.parse_fft_descriptor("scfo256x256*4")) is None
     assert bb.fused_configuration(bb.parse_fft_descriptor("scfo64x64*100")) is not None
@@ -97,5 +101,6 @@ def test_fusion_policy(monkeypatch):
     fr = bb.fused_configuration(c2r)
     assert fr["first_pass"] == 1 and fr["kb"] == 128 and fr["ka"] == 1
     assert bb.fused_configuration(bb.parse_fft_descriptor("scfo256x256*4")) is None
+    assert bb.fused_configuration(big)["lds_bytes"] > 96 * 1024
     monkeypatch.setenv("BBFFT_FUSE", "off")
     assert bb.fused_configuration(bb.parse_fft_descriptor("scfo64x64*100")) is None
