@@ -255,7 +255,8 @@ def _table_entries():
             continue
         lhs, rhs = line.split(":")
         prec, typ, N, M = lhs.split()
-        fac, Nt, Mb, Kb, pad, flags = rhs.split()
+        fac, Nt, Mb, Kb, pad, flags, *occupancy = rhs.split()  # [waves_per_eu [groups]]
+        assert len(occupancy) <= 2, line
         out.append((prec, typ, int(N), int(M), [int(f) for f in fac.split("x")], int(Nt),
                     int(Mb), int(Kb), int(pad), int(flags)))
     return out
