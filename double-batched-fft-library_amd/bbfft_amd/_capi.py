@@ -100,7 +100,6 @@ class CFusedInfo(ctypes.Structure):
         ("lds_bytes", ctypes.c_size_t), ("planes", ctypes.c_size_t), ("ka", ctypes.c_size_t),
         ("kb", ctypes.c_size_t), ("plane_elems", ctypes.c_size_t),
         ("in_plane_stride", ctypes.c_size_t), ("out_plane_stride", ctypes.c_size_t),
-        ("tiles_a", ctypes.c_size_t), ("tiles_b", ctypes.c_size_t),
     ]
 
 

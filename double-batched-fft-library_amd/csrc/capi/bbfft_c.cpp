@@ -541,8 +541,6 @@ int bbfft_fused_configuration(const bbfft_configuration *cfg, const char *device
             info->plane_elems = fp.fc.plane_elems;
             info->in_plane_stride = fp.fc.in_plane_stride;
             info->out_plane_stride = fp.fc.out_plane_stride;
-            info->tiles_a = fp.fc.tiles_a;
-            info->tiles_b = fp.fc.tiles_b;
         }
         if (*fused && name) {
             copy_out(fp.fc.identifier(), name, name_len);

@@ -317,15 +317,10 @@ class kernel_gen {
     /// alias_in / alias_out: the input / output pointer of a tile body points into the
     /// work-group's LDS scratch (fused multi-dimensional kernels), so reads of the input and
     /// writes of the output are separated from scratch writes / reads by barriers.
-    /// plane_stride: instead of a private scratch block, each tile exchanges through its own
-    /// columns of the LDS plane it reads (alias_in) or writes (alias_out), rows `plane_stride`
-    /// elements apart (c2c, one K column per tile, no padding); the other tiles' columns stay
-    /// untouched, so a plane can be processed as several M tiles in turn.
     kernel_gen(fft1d_configuration const &c, std::string name, std::size_t group_size = 0,
-               bool alias_in = false, bool alias_out = false, std::size_t plane_stride = 0)
+               bool alias_in = false, bool alias_out = false)
         : c_(c), f_{c.fp}, name_(std::move(name)), group_size_(group_size), alias_in_(alias_in),
-          alias_out_(alias_out), plane_stride_(plane_stride) {
-        rs_ = plane_stride ? plane_stride : c.Mb;
+          alias_out_(alias_out) {
         nfft_ = c.N_fft();
         L_ = c.factorization.size();
         real_ = c.type != transform_type::c2c;
@@ -380,9 +375,6 @@ class kernel_gen {
     std::string T() const { return f_.real(); }
     std::string T2() const { return f_.cplx(); }
 
-    /// A plane-scratch tiling whose last M tile is partial.
-    bool plane_partial() const { return plane_stride_ && MG_ * c_.Mb != c_.M; }
-
     /// Padded LDS index of FFT index n (expression).
     std::string padded(std::string const &n) const {
         return c_.lds_pad ? "(" + n + " + ((" + n + ") >> " + std::to_string(pad_shift_) + "))"
@@ -391,7 +383,7 @@ class kernel_gen {
     /// LDS slot at padded index `pbase` (a variable holding a padded index) plus a constant
     /// padded offset: the offset folds into the LDS instruction's immediate.
     std::string lds_slot(std::string const &pbase, std::size_t offset) const {
-        return "col[(" + pbase + " + " + u(offset) + ") * " + u(rs_) + "]";
+        return "col[(" + pbase + " + " + u(offset) + ") * " + u(c_.Mb) + "]";
     }
 
     /// LDS slot of FFT index n (expression) within this work-item's column.
@@ -399,7 +391,7 @@ class kernel_gen {
         std::string p = c_.lds_pad ? "(" + n + " + ((" + n + ") >> " +
                                          std::to_string(pad_shift_) + "))"
                                    : "(" + n + ")";
-        return "col[" + p + " * " + u(rs_) + "]";
+        return "col[" + p + " * " + u(c_.Mb) + "]";
     }
 
     std::size_t in_elem_bytes() const {
@@ -598,7 +590,7 @@ class kernel_gen {
         } else {
             w_.line("const unsigned long kcols = K;");
         }
-        if (use_lds_ && !plane_stride_) {
+        if (use_lds_) {
             std::string klc = band_.empty() ? "kl" : "(inb ? kl : 0u)";
             w_.line(T2() + "* col = lds_buf + ml + " + u(c_.Mb * np_) + " * " + klc + ";");
         }
@@ -610,15 +602,6 @@ class kernel_gen {
     /// One (M tile, K tile) of work: `tile` selects it.
     void emit_tile() {
         emit_tile_origin();
-        if (use_lds_ && plane_stride_) {
-            // the tile's own columns of the plane (one K column per tile); in a partial last M
-            // tile the work-items past M neither read nor write it (their slots belong to the
-            // next row)
-            w_.line(T2() + "* col = lds_buf + m0 + ml;");
-            if (MG_ * c_.Mb != c_.M) {
-                w_.line("const bool mv = m < " + ul(c_.M) + ";");
-            }
-        }
         if (c_.type == transform_type::c2r) {
             emit_c2r_pre();
             w_.line("__syncthreads();");
@@ -722,7 +705,7 @@ class kernel_gen {
         std::string q = c_.lds_pad ? "(" + n + " + ((" + n + ") >> " +
                                          std::to_string(pad_shift_) + "))"
                                    : "(" + n + ")";
-        return p + "[" + q + " * " + u(rs_) + "]";
+        return p + "[" + q + " * " + u(c_.Mb) + "]";
     }
 
     /**
@@ -961,9 +944,6 @@ class kernel_gen {
             } else if (!band_.empty()) {
                 cond = cond.empty() ? band_ : band_ + " && " + cond;
             }
-            if (!src_global && plane_partial()) {
-                cond = cond.empty() ? "mv" : "mv && " + cond;
-            }
             if (!cond.empty()) {
                 w_.open("if (" + cond + ")");
             }
@@ -1028,9 +1008,6 @@ class kernel_gen {
             std::string ccond = guard[b];
             if (!band_.empty()) {
                 ccond = ccond.empty() ? band_ : band_ + " && " + ccond;
-            }
-            if (!dst_global && plane_partial()) {
-                ccond = ccond.empty() ? "mv" : "mv && " + ccond;
             }
             if (!ccond.empty()) {
                 w_.open("if (" + ccond + ")");
@@ -1255,8 +1232,6 @@ class kernel_gen {
     bool half_in_ = false, bf16_ = false, mfma_ = false;
     bool local32_ = false;
     bool prefetch_ = false;
-    std::size_t plane_stride_ = 0;
-    std::size_t rs_ = 1; // LDS row stride (elements between FFT index n and n + 1)
 };
 
 } // namespace
@@ -1334,23 +1309,14 @@ void generate_fft1d_body(std::ostream &os, fft1d_configuration const &cfg,
 
 void generate_fft1d_body_aliased(std::ostream &os, fft1d_configuration const &cfg,
                                  std::string const &name, std::size_t group_size, bool alias_in,
-                                 bool alias_out, std::size_t plane_stride) {
+                                 bool alias_out) {
     if (cfg.load_function || cfg.store_function) {
         throw bad_configuration("callbacks are not supported in fused kernels");
     }
     if (cfg.flags & (fft1d_input_f16 | fft1d_input_bf16 | fft1d_mfma)) {
         throw bad_configuration("half-precision input is not supported in fused kernels");
     }
-    if (plane_stride) {
-        // the tile's scratch is its own plane columns: rows must hold the tile, one K column
-        if (cfg.type != transform_type::c2c || cfg.Kb != 1 || cfg.lds_pad != 0 ||
-            alias_in == alias_out || plane_stride < cfg.M ||
-            (alias_in ? cfg.istride[1] : cfg.ostride[1]) != plane_stride ||
-            (alias_in ? cfg.istride[0] : cfg.ostride[0]) != 1) {
-            throw bad_configuration("invalid plane scratch for a fused pass");
-        }
-    }
-    kernel_gen g(cfg, name, group_size, alias_in, alias_out, plane_stride);
+    kernel_gen g(cfg, name, group_size, alias_in, alias_out);
     os << g.generate_body();
 }
 

@@ -17,12 +17,10 @@ std::string hip_prelude();
 void generate_fft1d_body(std::ostream &os, fft1d_configuration const &cfg,
                          std::string const &name, std::size_t group_size);
 /// Tile body whose input (alias_in) or output (alias_out) pointer may point into the LDS
-/// scratch passed to the body (real transforms included). plane_stride > 0 (c2c, Kb = 1, no
-/// padding): the tile exchanges through its own columns of that LDS plane, whose rows are
-/// plane_stride elements apart, so the plane's M tiles can be processed one after another.
+/// scratch passed to the body (real transforms included).
 void generate_fft1d_body_aliased(std::ostream &os, fft1d_configuration const &cfg,
                                  std::string const &name, std::size_t group_size, bool alias_in,
-                                 bool alias_out, std::size_t plane_stride = 0);
+                                 bool alias_out);
 
 } // namespace bbfft
 

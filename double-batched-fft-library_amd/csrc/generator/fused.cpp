@@ -48,14 +48,11 @@ void factorizations(std::size_t n, std::size_t rmax, std::size_t parts, std::vec
     }
 }
 
-/// Kernel configuration of pass `p` restricted to one plane (`kcols` K-columns). With
-/// `max_tiles` == 1 the plane is one tile: the factorisation with the smallest largest radix
-/// (fewest registers per work-item) whose work-group fits `wg_max`. A c2c pass with one K
-/// column may instead split M into up to `max_tiles` tiles (exchanging through their own plane
-/// columns): the fewest tiles whose largest radix is at most `radix_cap`. False if none fits.
-bool pass_tiling(configuration p, std::size_t kcols, device_info const &info, unsigned flags,
-                 unsigned variant, std::size_t wg_max, std::size_t max_tiles,
-                 std::size_t radix_cap, fft1d_configuration &out, std::size_t &tiles) {
+/// Kernel configuration of pass `p` restricted to one plane (`kcols` K-columns, one tile):
+/// the factorisation with the smallest largest radix (fewest registers per work-item) whose
+/// work-group fits `wg_max`; false if none fits.
+bool one_tile(configuration p, std::size_t kcols, device_info const &info, unsigned flags,
+              unsigned variant, std::size_t wg_max, fft1d_configuration &out) {
     p.shape[2] = kcols;
     std::size_t const M = p.shape[0];
     bool const real = p.type != transform_type::c2c;
@@ -69,51 +66,35 @@ bool pass_tiling(configuration p, std::size_t kcols, device_info const &info, un
         cands.push_back({static_cast<int>(nfft)});
     }
     factorizations(nfft, rmax, 3, cur, cands);
-    bool const split = max_tiles > 1 && !real && cols == 1 && M > 1;
     std::vector<int> const *best = nullptr;
-    std::size_t best_key = ~std::size_t(0), best_mb = M;
-    for (std::size_t nt = 1; nt <= (split ? max_tiles : 1); ++nt) {
-        std::size_t const mb = (M + nt - 1) / nt;
-        if (nt > 1 && (M + mb - 1) / mb != nt) {
-            continue; // the same M tile as a smaller count
+    std::size_t best_key = ~std::size_t(0);
+    for (auto const &f : cands) {
+        std::size_t const r = static_cast<std::size_t>(f.front()); // largest radix
+        if (M * (nfft / r) * cols > wg_max) {
+            continue;
         }
-        for (auto const &f : cands) {
-            std::size_t const r = static_cast<std::size_t>(f.front()); // largest radix
-            if (mb * (nfft / r) * cols > wg_max) {
-                continue;
-            }
-            // split: within the radix cap, fewest tiles first; then smallest largest radix,
-            // fewest stages, most balanced
-            std::size_t const over = split && r > radix_cap ? 1 : 0;
-            std::size_t const key = ((over * 16 + (split ? nt : 0)) * 256 + r) * 64 +
-                                    f.size() * 16 + (r - static_cast<std::size_t>(f.back()));
-            if (key < best_key) {
-                best_key = key;
-                best = &f;
-                best_mb = mb;
-            }
+        // smallest largest radix, then fewest stages, then most balanced
+        std::size_t const key = r * 64 + f.size() * 16 + (r - static_cast<std::size_t>(f.back()));
+        if (key < best_key) {
+            best_key = key;
+            best = &f;
         }
     }
-    if (!best || (split && static_cast<std::size_t>(best->front()) > radix_cap &&
-                  best_mb != M)) {
+    if (!best) {
         return false;
     }
     fft1d_tuning t;
     t.factorization = *best;
     t.Nt = nfft / static_cast<std::size_t>(best->front());
-    t.Mb = best_mb;
+    t.Mb = M;
     t.Kb = cols;
     t.flags = flags;
     t.has_flags = true;
-    if (best_mb < M) {
-        t.no_pad = true; // the scratch is the plane itself
-    }
     try {
         out = configure_fft1d(p, info, t, variant);
     } catch (bad_configuration const &) {
         return false;
     }
-    tiles = (M + best_mb - 1) / best_mb;
     return true;
 }
 
@@ -125,10 +106,7 @@ std::string fused2_configuration::identifier() const {
         << std::hex << std::setw(16) << std::setfill('0')
         << fnv1a64(a.identifier() + ";" + b.identifier() + ";" + std::to_string(ka) + ";" +
                    std::to_string(kb) + ";" + std::to_string(plane_elems) + ";" +
-                   std::to_string(in_plane_stride) + ";" + std::to_string(out_plane_stride) +
-                   ";" + std::to_string(tiles_a) + ";" + std::to_string(tiles_b) + ";" +
-                   std::to_string(plane_stride_a) + ";" + std::to_string(plane_stride_b) + ";" +
-                   std::to_string(waves_per_eu));
+                   std::to_string(in_plane_stride) + ";" + std::to_string(out_plane_stride));
     return oss.str();
 }
 
@@ -162,41 +140,22 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
     if (pa.shape[2] % ka || pb.shape[2] % kb || pa.shape[2] / ka != pb.shape[2] / kb) {
         return false;
     }
-    // Streaming hints on the global side only. First choice (fp32): 512-work-item groups with
-    // at most 16 elements per work-item (c2c passes over several M tiles if needed) and at
-    // most half the LDS, compiled for 4 waves per SIMD, so that two planes share a CU and one
-    // plane's memory phase overlaps the other's LDS stages. Otherwise as many work-items as the
-    // device allows (one tile per pass, smallest radices), typically one plane per CU.
-    auto attempt = [&](std::size_t wg_max, std::size_t max_tiles, std::size_t radix_cap,
-                       fused2_configuration &fc) {
-        fc = {};
-        if (!pass_tiling(pa, ka, info, fft1d_nontemporal_load, variant_a, wg_max, max_tiles,
-                         radix_cap, fc.a, fc.tiles_a) ||
-            !pass_tiling(pb, kb, info, fft1d_nontemporal_store, variant_b, wg_max, max_tiles,
-                         radix_cap, fc.b, fc.tiles_b)) {
-            return false;
-        }
-        fc.ka = ka;
-        fc.kb = kb;
-        fc.planes = pa.shape[2] / ka;
-        fc.plane_elems = P;
-        fc.in_plane_stride = ka * pa.istride[2];
-        fc.out_plane_stride = kb * pb.ostride[2];
-        fc.plane_stride_a = fc.tiles_a > 1 ? fc.a.ostride[1] : 0;
-        fc.plane_stride_b = fc.tiles_b > 1 ? fc.b.istride[1] : 0;
-        fc.group_size = std::max(fc.a.work_group_size(), fc.b.work_group_size());
-        return fc.lds_bytes() <= info.local_memory_size;
-    };
+    fused2_configuration fc = {};
+    // As many work-items as the device allows, so each holds few elements (smallest radices);
+    // streaming hints on the global side only.
     std::size_t const wg_max = std::min<std::size_t>(1024, info.max_work_group_size);
-    fused2_configuration fc;
-    if (pa.fp == precision::f32 && wg_max >= 512 && attempt(512, 4, 8, fc) &&
-        std::max(fc.a.factorization.front(), fc.b.factorization.front()) <= 16 &&
-        2 * fc.lds_bytes() <= info.local_memory_size) {
-        fc.waves_per_eu = 4;
-        out = fc;
-        return true;
+    if (!one_tile(pa, ka, info, fft1d_nontemporal_load, variant_a, wg_max, fc.a) ||
+        !one_tile(pb, kb, info, fft1d_nontemporal_store, variant_b, wg_max, fc.b)) {
+        return false;
     }
-    if (!attempt(wg_max, 1, 0, fc)) {
+    fc.ka = ka;
+    fc.kb = kb;
+    fc.planes = pa.shape[2] / ka;
+    fc.plane_elems = P;
+    fc.in_plane_stride = ka * pa.istride[2];
+    fc.out_plane_stride = kb * pb.ostride[2];
+    fc.group_size = std::max(fc.a.work_group_size(), fc.b.work_group_size());
+    if (fc.lds_bytes() > info.local_memory_size) {
         return false;
     }
     out = fc;
@@ -214,33 +173,17 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
     // body names derive from the kernel name: several kernels may share one source
     std::string const fa = n + "_a", fb = n + "_b";
     os << hip_prelude();
-    generate_fft1d_body_aliased(os, fc.a, fa, fc.group_size, false, true, fc.plane_stride_a);
-    generate_fft1d_body_aliased(os, fc.b, fb, fc.group_size, true, false, fc.plane_stride_b);
-    std::string const occupancy =
-        fc.waves_per_eu ? " __attribute__((amdgpu_waves_per_eu(" +
-                              std::to_string(fc.waves_per_eu) + ")))"
-                        : "";
-    // M tiles of a pass touch disjoint plane columns, so they follow one another without
-    // barriers beyond the ones inside the tile bodies.
-    auto calls = [&](std::string const &f, std::size_t tiles, std::string const &args) {
-        std::ostringstream c;
-        for (std::size_t t = 0; t < tiles; ++t) {
-            c << "    " << f << "(" << args << ", " << t << "ul, lds_buf, threadIdx.x);\n";
-        }
-        return c.str();
-    };
-    os << "extern \"C\" __global__ void __launch_bounds__(" << fc.group_size << ")"
-       << occupancy << " " << n << "(const " << in_t << "* in, " << out_t
-       << "* out, unsigned long planes) {\n"
+    generate_fft1d_body_aliased(os, fc.a, fa, fc.group_size, false, true);
+    generate_fft1d_body_aliased(os, fc.b, fb, fc.group_size, true, false);
+    os << "extern \"C\" __global__ void __launch_bounds__(" << fc.group_size << ") " << n
+       << "(const " << in_t << "* in, " << out_t << "* out, unsigned long planes) {\n"
        << "  __shared__ " << T2 << " lds_buf[" << lds_elems << "];\n"
        << "  for (unsigned long f = blockIdx.x; f < planes; f += gridDim.x) {\n"
-       << calls(fa, fc.tiles_a,
-                "in + f * " + std::to_string(fc.in_plane_stride) + "ul, lds_buf, " +
-                    std::to_string(fc.ka) + "ul")
+       << "    " << fa << "(in + f * " << fc.in_plane_stride << "ul, lds_buf, " << fc.ka
+       << "ul, 0ul, lds_buf, threadIdx.x);\n"
        << "    __syncthreads();\n"
-       << calls(fb, fc.tiles_b,
-                "lds_buf, out + f * " + std::to_string(fc.out_plane_stride) + "ul, " +
-                    std::to_string(fc.kb) + "ul")
+       << "    " << fb << "(lds_buf, out + f * " << fc.out_plane_stride << "ul, " << fc.kb
+       << "ul, 0ul, lds_buf, threadIdx.x);\n"
        << "    __syncthreads();\n"
        << "  }\n"
        << "}\n";

@@ -149,12 +149,6 @@ struct BBFFT_EXPORT fused2_configuration {
     std::size_t in_plane_stride;    ///< input elements between planes (ka * a.istride[2])
     std::size_t out_plane_stride;   ///< output elements between planes (kb * b.ostride[2])
     std::size_t group_size;         ///< work-items per work-group
-    /// M tiles of pass a / pass b per plane (1: the whole plane in one tile). A c2c pass with
-    /// several tiles exchanges through its own columns of the plane (rows plane_stride_a /
-    /// plane_stride_b elements apart; 0 = private scratch, single tile).
-    std::size_t tiles_a = 1, tiles_b = 1;
-    std::size_t plane_stride_a = 0, plane_stride_b = 0;
-    unsigned waves_per_eu = 0; ///< occupancy target of the compiler (0: its default)
 
     std::string identifier() const;
     std::size_t lds_bytes() const;

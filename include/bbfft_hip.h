@@ -204,7 +204,6 @@ typedef struct bbfft_fused_info {
     unsigned first_pass;
     size_t group_size, lds_bytes, planes, ka, kb, plane_elems, in_plane_stride,
         out_plane_stride;
-    size_t tiles_a, tiles_b; /* M tiles per plane of the first / second fused pass */
 } bbfft_fused_info;
 
 BBFFT_EXPORT_C int bbfft_fused_configuration(const bbfft_configuration *cfg,

@@ -64,31 +64,6 @@ def test_fused_vs_oracle(fp, t, M, dims, K):
     assert oracle.rel_l2(y, ref) <= oracle.tolerance(fp, int(np.prod(dims)))
 
 
-@pytest.mark.parametrize("t,M,dims,K", [(1, 1, [128, 64], 2), (2, 1, [128, 64], 2),
-                                        (0, 1, [64, 128], 2), (0, 1, [96, 80], 1),
-                                        (1, 1, [128, 128, 2], 1), (2, 1, [128, 128, 2], 1)])
-def test_fused_plane_tiles_vs_oracle(t, M, dims, K):
-    """Fused c2c passes split into M tiles that exchange through their own plane columns
-    (512-work-item groups; the 65-column plane of an N=128 real mode gives a partial last
-    tile)."""
-    cfg = _cfg(M, dims, K, 4, t)
-    fc = bb.fused_configuration(cfg)
-    assert fc is not None and fc["group_size"] <= 512
-    assert max(fc["tiles_a"], fc["tiles_b"]) > 1
-    shape = cfg.shape
-    D = len(dims)
-    half = list(shape)
-    half[1] = dims[0] // 2 + 1
-    ishape, oshape = (shape, half) if t == 1 else (half, shape) if t == 2 else (shape, shape)
-    n_in = oracle.extent(D, ishape, cfg.istride)
-    n_out = oracle.extent(D, oshape, cfg.ostride)
-    x = oracle.uniform_real(7, n_in) if t == 1 else oracle.uniform_complex(7, n_in)
-    x = x.astype(np.float32 if t == 1 else np.complex64)
-    ref = oracle.fft(D, shape, cfg.istride, cfg.ostride, t, int(cfg.dir), x, out_size=n_out)
-    y = _run_plan(cfg, x, n_out)
-    assert oracle.rel_l2(y, ref) <= oracle.tolerance(4, int(np.prod(dims)))
-
-
 def test_fused_in_place_layout_r2c():
     """In-place r2c layout (padded real rows) read by the fused kernel."""
     cfg = _cfg(1, [16, 8, 4], 2, 4, 1, inplace=True)
