@@ -138,7 +138,6 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     c.load_function = cfg.callbacks.load_function;
     c.store_function = cfg.callbacks.store_function;
     std::size_t const nfft = c.N_fft();
-    std::size_t const real_bytes = static_cast<std::size_t>(cfg.fp);
 
     if (half_in) {
         if ((all_flags & fft1d_input_f16) && (all_flags & fft1d_input_bf16)) {
