@@ -3,11 +3,13 @@
 Workload (headline): plain c2c forward, fp32, M=8 x N=256 x K=32768 (descriptor
 "scfo8.256*32768"), out-of-place, inputs uniform[-1,1) resident in HBM. A step is one
 plan.execute over the whole tensor. Also reported: the sweep N in {16..512, 60, 105, 243, 384}
-for fp32 and fp64 (same M, K), a device-to-device copy bandwidth, and the CPU baseline (the
-oracle's C port of the reference algorithm on a bounded sample, host cores), and the ragged
-serving workload (4096 mixed jobs in one persistent launch vs per-job plain launches) and the
-N=512 precision variant (fp16 / bf16 storage with the MFMA-split radix-16 stage, errors against
-an FP64 FFT of the same values).
+for fp32 and fp64 (same M, K), device-to-device copy bandwidths (torch's copy and hand-written
+streaming-copy kernels: the measured ceiling), the CPU baseline (the oracle's C port of the
+reference algorithm on a bounded sample, host cores), the 32 x 128^3 real 3-D transforms
+(fused and unfused) and their differentiated filter pipeline (forward vs backward time), the
+ragged serving workload (4096 mixed jobs in one graph-replayed launch vs per-job plain
+launches) and the N=512 precision variant (fp16 / bf16 storage with the MFMA-split radix-16
+stage, errors against an FP64 FFT of the same values).
 
 Accounting (reference benchmark/adapter.hpp:51-52): flops = 5 N log2 N M K, bytes = 2 M N K
 sizeof(complex). Timing: HIP events on the plan's stream around the timed steps.
@@ -270,6 +272,44 @@ def nd_real_3d(bb, torch, cache, steps):
     return out
 
 
+def autograd_3d_filter(torch, steps):
+    """Differentiated 3-D pipeline on 32 x 128^3 fp32 (normal inputs): r2c, multiply by a
+    seeded complex 128 x 128 x 65 filter, c2r / 128^3, loss = sum of squares; forward and
+    backward (hand-written adjoint kernels through torch.autograd) timed separately with HIP
+    events on the current stream."""
+    from bbfft_amd import autograd as ag
+    n, B = 128, 32
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    x = torch.randn(B, n, n, n, generator=g, device="cuda", requires_grad=True)
+    f = torch.complex(torch.randn(n, n, n // 2 + 1, generator=g, device="cuda"),
+                      torch.randn(n, n, n // 2 + 1, generator=g, device="cuda"))
+
+    def forward():
+        z = ag.irfftn(ag.rfftn(x, 3) * f, n, 3) / n ** 3
+        return (z * z).sum()
+
+    for _ in range(2):
+        torch.autograd.grad(forward(), x)
+    torch.cuda.synchronize()
+    reps = max(3, min(20, steps // 10))
+    tf = tb = 0.0
+    for _ in range(reps):
+        s, m, e = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        s.record()
+        loss = forward()
+        m.record()
+        torch.autograd.grad(loss, x)
+        e.record()
+        torch.cuda.synchronize()
+        tf += s.elapsed_time(m)
+        tb += m.elapsed_time(e)
+    del x, f
+    torch.cuda.empty_cache()
+    return {"forward_ms": round(tf / reps, 4), "backward_ms": round(tb / reps, 4),
+            "backward_over_forward": round(tb / tf, 3), "reps": reps}
+
+
 def cpu_baseline(budget_s=15.0):
     """Oracle port of the reference algorithm on host cores, bounded sample of the workload."""
     try:
@@ -290,6 +330,7 @@ def main():
     ap.add_argument("--no-ragged", action="store_true")
     ap.add_argument("--no-mfma", action="store_true")
     ap.add_argument("--no-nd", action="store_true")
+    ap.add_argument("--no-autograd", action="store_true")
     ap.add_argument("--desc", default="scfo8.256*32768")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (counter calibration)")
@@ -322,6 +363,10 @@ def main():
     if not a.no_nd:
         nd = nd_real_3d(bb, torch, cache, a.steps)
         print(json.dumps({"nd_real_3d": nd}), file=sys.stderr, flush=True)
+    grad = None
+    if not a.no_autograd:
+        grad = autograd_3d_filter(torch, a.steps)
+        print(json.dumps({"autograd_3d_filter": grad}), file=sys.stderr, flush=True)
     mf = None
     if not a.no_mfma:
         mf = mfma_split(bb, torch, cache, a.steps)
@@ -354,7 +399,8 @@ def main():
                    "frac_of_stream_copy": (round(head["GBps"] / max(stream.values()), 4)
                                            if stream else None),
                    "kernel": head["kernel"]["kernel"],
-                   "sweep": sweep, "nd_real_3d_32x128cubed": nd, "mfma_split_n512": mf,
+                   "sweep": sweep, "nd_real_3d_32x128cubed": nd,
+                   "autograd_3d_filter_32x128cubed": grad, "mfma_split_n512": mf,
                    "ragged_serving": rag},
         "roofline": {"bound": "hbm", "achieved": round(head["GBps"], 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(head["frac"], 4),
