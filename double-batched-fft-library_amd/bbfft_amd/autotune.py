@@ -6,10 +6,14 @@ hints for each (precision, transform type, N, M), and writes the table the libra
     python -m bbfft_amd.autotune [-o TABLE] [--rounds 2] DESCRIPTOR...
 
 Search: coordinate descent from the library's heuristic default, one parameter group at a time
-(factorisation + work-items, K tile, M tile, LDS padding, flags), keeping the fastest. Every
+(factorisation + work-items, K tile, M tile, LDS padding, streaming hints, occupancy target,
+persistent grid with and without the load-first tile loop), keeping the fastest. Every
 candidate's output is compared with the default plan's output on the same input (relative L2
 within the precision's tolerance) before it is timed; the default plan itself is covered by the
 GPU parity suite against the FP64 oracle. Time = best of 3 runs of `reps` launches (HIP events).
+Confirmation (`--confirm ROUNDS`, default 6): the 4 fastest candidates are re-timed in turn,
+round after round, and the best median wins, so that a few per cent of drift between the
+search's measurements does not decide close calls.
 """
 import argparse
 import itertools
@@ -229,6 +233,34 @@ class Tuner:
             best, best_ms = self._try(best, best_ms, opts)
         return best, best_ms
 
+    def confirm(self, top=4, rounds=6):
+        """Re-decide between the `top` fastest distinct candidates of the search by interleaved
+        timing in this process: `rounds` rounds, each timing every candidate once in turn, so
+        slow drifts of the device (clocks, temperature) hit all of them alike. Returns the
+        candidate with the best median and that median (ms)."""
+        import statistics
+        torch = self.torch
+        ranked = sorted((ms, k) for k, ms in self.results.items() if ms < float("inf"))[:top]
+        plans = [(json.loads(k), make_plan(self.cfg, cache=self.cache, tuning=json.loads(k)))
+                 for _, k in ranked]
+        times = [[] for _ in plans]
+        for _ in range(rounds):
+            for i, (_, plan) in enumerate(plans):
+                plan.execute(self.x, self.y)
+                s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                s.record()
+                for _ in range(self.reps):
+                    plan.execute(self.x, self.y)
+                e.record()
+                torch.cuda.synchronize()
+                times[i].append(s.elapsed_time(e) / self.reps)
+        med = [statistics.median(t) for t in times]
+        for (t, _), m in zip(plans, med):
+            self.log(desc=self.desc, confirm=t, median_ms=round(m, 5),
+                     GBps=round(self.bytes / m / 1e6, 1))
+        i = min(range(len(plans)), key=lambda j: med[j])
+        return plans[i][0], med[i]
+
     def _try(self, best, best_ms, options, complete=False):
         for o in options:
             t = self.full_tuning(o) if complete else o
@@ -254,7 +286,7 @@ def table_line(cfg, tuning, default_ms, best_ms, nbytes, desc):
     groups = tuning.get("groups_per_cu", 0)
     extra = f" {waves} {groups}" if groups else (f" {waves}" if waves else "")
     return (f"{prec} {typ} {cfg.shape[1]} {cfg.shape[0]} : {fac} {tuning['Nt']} {tuning['Mb']} "
-            f"{tuning['Kb']} {tuning['lds_pad']} {flags & 3}{extra}   # "
+            f"{tuning['Kb']} {tuning['lds_pad']} {flags & (3 | PREFETCH)}{extra}   # "
             f"{nbytes / default_ms / 1e6:.0f} -> {nbytes / best_ms / 1e6:.0f} GB/s, {desc}")
 
 
@@ -291,6 +323,9 @@ def main(argv=None):
     ap.add_argument("--storage", choices=["fp32", "f16", "bf16"], default="fp32",
                     help="input storage of c2c fp32 descriptors (half-input kernels)")
     ap.add_argument("--mfma", action="store_true", help="with --storage f16/bf16: MFMA stage")
+    ap.add_argument("--confirm", type=int, default=6, metavar="ROUNDS",
+                    help="re-time the 4 fastest candidates interleaved over ROUNDS rounds and "
+                         "keep the best median (0: keep the search's best)")
     ap.add_argument("descriptors", nargs="+")
     a = ap.parse_args(argv)
     lines = []
@@ -301,6 +336,8 @@ def main(argv=None):
             variant |= 32
         tuner = Tuner(desc, reps=a.reps, variant=variant)
         best, best_ms = tuner.run(a.rounds)
+        if a.confirm > 0:
+            best, best_ms = tuner.confirm(rounds=a.confirm)
         line = table_line(tuner.cfg, best, tuner.default_ms, best_ms, tuner.bytes, desc)
         lines.append(line)
         print(json.dumps({"desc": desc, "best": best, "ms": round(best_ms, 5),

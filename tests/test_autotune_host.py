@@ -72,6 +72,11 @@ def test_merge_and_line_format(tmp_path):
     t = {"factorization": [8, 8, 4], "Nt": 32, "Mb": 8, "Kb": 1, "lds_pad": 16, "flags": 3}
     line = autotune.table_line(cfg, t, 1.0, 0.5, 1e9, "scfo8.256*32768")
     assert re.match(r"f32 c2c 256 8 : 8x8x4 32 8 1 16 3 +# 1000 -> 2000 GB/s", line)
+    # the load-first tile loop flag and the occupancy columns survive; storage bits move to
+    # the type
+    t = dict(t, flags=bb.PREFETCH | 3 | bb.INPUT_F16, waves_per_eu=0, groups_per_cu=2)
+    line = autotune.table_line(cfg, t, 1.0, 0.5, 1e9, "scfo8.256*32768")
+    assert re.match(r"f32 c2c-f16 256 8 : 8x8x4 32 8 1 16 67 0 2 +# 1000 -> 2000 GB/s", line)
 
 
 def test_benchmark_cli_k_selection():
