@@ -198,14 +198,17 @@ class Tuner:
         best, best_ms = dict(base), self.default_ms
         kcols = (self.K + 1) // 2 if (self.type != 0 and self.N % 2 == 1) else self.K
         for _ in range(rounds):
-            # 1) factorisation and work-items per transform (tile re-derived by the library)
+            # 1) factorisation and work-items per transform (tile re-derived by the library),
+            #    also with a one-column K tile: small groups (down to a single wave, which
+            #    needs no barriers) can win even where the default K tile does not
             opts = []
             for f in self.factorizations():
                 nt0 = self.nfft // max(f[1:] if self.variant & 32 else f)
                 for nt in sorted({nt0, 2 * nt0}):
                     if nt <= 256:
-                        opts.append({"factorization": f, "Nt": nt, "flags": best["flags"],
-                                     "lds_pad": best["lds_pad"], "no_pad": best["no_pad"]})
+                        o = {"factorization": f, "Nt": nt, "flags": best["flags"],
+                             "lds_pad": best["lds_pad"], "no_pad": best["no_pad"]}
+                        opts += [o, dict(o, Kb=1)]
             best, best_ms = self._try(best, best_ms, opts, complete=True)
             # 2) K tile
             opts = [dict(best, Kb=kb) for kb in _pow2_upto(min(64, kcols))
