@@ -534,7 +534,7 @@ int bbfft_fused_configuration(const bbfft_configuration *cfg, const char *device
         if (*fused && info) {
             info->first_pass = static_cast<unsigned>(fp.first);
             info->group_size = fp.fc.group_size;
-            info->lds_bytes = fp.fc.lds_bytes();
+            info->lds_bytes = fp.fc.lds_bytes() + 4 * fp.fc.stage_words;
             info->planes = fp.fc.planes;
             info->ka = fp.fc.ka;
             info->kb = fp.fc.kb;

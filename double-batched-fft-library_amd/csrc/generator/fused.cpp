@@ -8,6 +8,7 @@
 #include "heuristics.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <iomanip>
 #include <ostream>
 #include <sstream>
@@ -106,7 +107,8 @@ std::string fused2_configuration::identifier() const {
         << std::hex << std::setw(16) << std::setfill('0')
         << fnv1a64(a.identifier() + ";" + b.identifier() + ";" + std::to_string(ka) + ";" +
                    std::to_string(kb) + ";" + std::to_string(plane_elems) + ";" +
-                   std::to_string(in_plane_stride) + ";" + std::to_string(out_plane_stride));
+                   std::to_string(in_plane_stride) + ";" + std::to_string(out_plane_stride) +
+                   ";" + std::to_string(stage_words));
     return oss.str();
 }
 
@@ -158,6 +160,27 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
     if (fc.lds_bytes() > info.local_memory_size) {
         return false;
     }
+    // Staging: with the plane's raw input in a second LDS buffer, the next plane's input loads
+    // (direct global-to-LDS, 4 bytes per work-item and instruction) run while the current
+    // plane's second pass computes and stores. Needs room for both buffers; BBFFT_FUSE_STAGE=off
+    // disables it.
+    char const *stage_env = std::getenv("BBFFT_FUSE_STAGE");
+    bool const stage_off = stage_env && std::string(stage_env) == "off";
+    std::size_t const in_bytes = pa.type == transform_type::r2c
+                                     ? static_cast<std::size_t>(pa.fp)
+                                     : 2 * static_cast<std::size_t>(pa.fp);
+    std::size_t const words = (fc.in_plane_stride * in_bytes / 4 + 63) / 64 * 64;
+    if (!stage_off && info.type == device_type::gpu &&
+        fc.lds_bytes() + 4 * words <= info.local_memory_size) {
+        fused2_configuration st = fc;
+        // pass a now reads LDS: no streaming hint on its loads
+        if (one_tile(pa, ka, info, 0u, variant_a, wg_max, st.a) &&
+            st.a.work_group_size() == fc.a.work_group_size() &&
+            st.lds_bytes() + 4 * words <= info.local_memory_size) {
+            st.stage_words = words;
+            fc = st;
+        }
+    }
     out = fc;
     return true;
 }
@@ -173,15 +196,64 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
     // body names derive from the kernel name: several kernels may share one source
     std::string const fa = n + "_a", fb = n + "_b";
     os << hip_prelude();
+    if (fc.stage_words) {
+        // direct global-to-LDS load of one 32-bit word per work-item (the LDS address is the
+        // wave-uniform base plus 4 bytes per lane) and a wait for all outstanding loads
+        os << "#ifndef BBFFT_LDS_DMA4\n"
+              "#define BBFFT_LDS_DMA4(src, dst) __builtin_amdgcn_global_load_lds("
+              "(const __attribute__((address_space(1))) void*)(src), "
+              "(__attribute__((address_space(3))) void*)(dst), 4, 0, 0)\n"
+              "#define BBFFT_WAIT_ALL() __builtin_amdgcn_s_waitcnt(0)\n"
+              "#endif\n";
+    }
     generate_fft1d_body_aliased(os, fc.a, fa, fc.group_size, false, true);
     generate_fft1d_body_aliased(os, fc.b, fb, fc.group_size, true, false);
     os << "extern \"C\" __global__ void __launch_bounds__(" << fc.group_size << ") " << n
        << "(const " << in_t << "* in, " << out_t << "* out, unsigned long planes) {\n"
-       << "  __shared__ " << T2 << " lds_buf[" << lds_elems << "];\n"
-       << "  for (unsigned long f = blockIdx.x; f < planes; f += gridDim.x) {\n"
-       << "    " << fa << "(in + f * " << fc.in_plane_stride << "ul, lds_buf, " << fc.ka
+       << "  __shared__ " << T2 << " lds_buf[" << lds_elems << "];\n";
+    if (!fc.stage_words) {
+        os << "  for (unsigned long f = blockIdx.x; f < planes; f += gridDim.x) {\n"
+           << "    " << fa << "(in + f * " << fc.in_plane_stride << "ul, lds_buf, " << fc.ka
+           << "ul, 0ul, lds_buf, threadIdx.x);\n"
+           << "    __syncthreads();\n"
+           << "    " << fb << "(lds_buf, out + f * " << fc.out_plane_stride << "ul, " << fc.kb
+           << "ul, 0ul, lds_buf, threadIdx.x);\n"
+           << "    __syncthreads();\n"
+           << "  }\n"
+           << "}\n";
+        return;
+    }
+    // Staged: plane f's raw input (W words) sits in `stage` before pass a starts; the copy of
+    // plane f + gridDim.x is issued once pass a has consumed `stage` and overlaps pass b.
+    // Each wave copies blocks of 64 words: lane l's word lands at the block base + l.
+    std::size_t const in_bytes = fc.a.type == transform_type::r2c
+                                     ? static_cast<std::size_t>(fc.a.fp)
+                                     : 2 * static_cast<std::size_t>(fc.a.fp);
+    std::size_t const W = fc.in_plane_stride * in_bytes / 4;
+    // Blocks of BS words, one per copying wave and step: the full waves copy 64 words each;
+    // a group smaller than one wave copies group_size words per step with its only wave.
+    std::size_t const BS = std::min<std::size_t>(64, fc.group_size);
+    std::size_t const NWc = fc.group_size >= 64 ? fc.group_size / 64 : 1;
+    std::size_t const NB = (W + BS - 1) / BS;
+    os << "  __shared__ float stage[" << fc.stage_words << "];\n"
+       << "  const unsigned wv = threadIdx.x / 64u, ln = threadIdx.x % 64u;\n"
+       << "  auto stage_copy = [&](unsigned long f) {\n"
+       << "    const float* src = (const float*)in + f * " << W << "ul;\n"
+       << "    if (wv >= " << NWc << "u) return; // a partial last wave copies nothing\n"
+       << "    for (unsigned b = wv; b < " << NB << "u; b += " << NWc << "u) {\n"
+       << "      const unsigned i = b * " << BS << "u + ln;\n"
+       << "      if (i < " << W << "u) BBFFT_LDS_DMA4(src + i, stage + b * " << BS << "u);\n"
+       << "    }\n"
+       << "  };\n"
+       << "  unsigned long f = blockIdx.x;\n"
+       << "  if (f < planes) stage_copy(f);\n"
+       << "  for (; f < planes; f += gridDim.x) {\n"
+       << "    BBFFT_WAIT_ALL();\n"
+       << "    __syncthreads();\n"
+       << "    " << fa << "((const " << in_t << "*)stage, lds_buf, " << fc.ka
        << "ul, 0ul, lds_buf, threadIdx.x);\n"
        << "    __syncthreads();\n"
+       << "    if (f + gridDim.x < planes) stage_copy(f + gridDim.x);\n"
        << "    " << fb << "(lds_buf, out + f * " << fc.out_plane_stride << "ul, " << fc.kb
        << "ul, 0ul, lds_buf, threadIdx.x);\n"
        << "    __syncthreads();\n"

@@ -133,6 +133,17 @@ fused2_kernel::fused2_kernel(fused2_configuration const &fc, hipStream_t stream,
     }
     fn_ = hip::create_kernel(reinterpret_cast<hipModule_t>(mod_.get()), name_);
     grid_ = static_cast<unsigned>(std::min<std::size_t>(fc_.planes, std::size_t(1) << 24));
+    if (fc_.stage_words) {
+        // staged kernels overlap planes inside a work-group: one resident group per slot
+        int per_cu = 0, cus = 0;
+        BBFFT_HIP_CHECK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, fn_, static_cast<int>(fc_.group_size), 0));
+        BBFFT_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                              device));
+        std::size_t const resident =
+            static_cast<std::size_t>(std::max(per_cu, 1)) * static_cast<std::size_t>(std::max(cus, 1));
+        grid_ = static_cast<unsigned>(std::min<std::size_t>(fc_.planes, resident));
+    }
 }
 
 void fused2_kernel::launch(void const *in, void *out) {
@@ -152,7 +163,8 @@ void fused2_kernel::launch(void const *in, void *out) {
 }
 
 launch_info fused2_kernel::info() const {
-    return launch_info{name_, grid_, static_cast<unsigned>(fc_.group_size), fc_.lds_bytes()};
+    return launch_info{name_, grid_, static_cast<unsigned>(fc_.group_size),
+                       fc_.lds_bytes() + 4 * fc_.stage_words};
 }
 
 nd_plan::nd_plan(pass_layout const &layout, hipStream_t stream, int device, jit_cache *cache,

@@ -149,9 +149,13 @@ struct BBFFT_EXPORT fused2_configuration {
     std::size_t in_plane_stride;    ///< input elements between planes (ka * a.istride[2])
     std::size_t out_plane_stride;   ///< output elements between planes (kb * b.ostride[2])
     std::size_t group_size;         ///< work-items per work-group
+    /// 32-bit words of one plane's input (rounded up to 64) that each work-group streams into
+    /// an LDS staging buffer with direct global-to-LDS loads while it still works on the
+    /// previous plane (persistent grid); 0: pass a reads global memory itself.
+    std::size_t stage_words = 0;
 
     std::string identifier() const;
-    std::size_t lds_bytes() const;
+    std::size_t lds_bytes() const; ///< plane and stage scratch (without the staging buffer)
 };
 
 /**

@@ -31,6 +31,9 @@ struct emu_dim3 {
 #define __shared__ static
 #define __launch_bounds__(x)
 #define __builtin_nontemporal_load(p) (*(p))
+// direct global-to-LDS load: the work-item's word lands at the wave's LDS base + its lane
+#define BBFFT_LDS_DMA4(src, dst) (((float*)(dst))[threadIdx.x % 64u] = *(const float*)(src))
+#define BBFFT_WAIT_ALL() ((void)0)
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))
 
 namespace emu {

@@ -64,6 +64,35 @@ def test_fused_vs_oracle(fp, t, M, dims, K):
     assert oracle.rel_l2(y, ref) <= oracle.tolerance(fp, int(np.prod(dims)))
 
 
+@pytest.mark.parametrize("t,M,dims,K", [(0, 1, [16, 12], 5), (1, 1, [16, 12], 5),
+                                        (2, 1, [16, 12], 5), (1, 2, [8, 6, 5], 3),
+                                        (2, 1, [10, 8, 4], 3), (0, 3, [6, 9], 4)])
+def test_fused_staged_matches_unstaged(t, M, dims, K, monkeypatch):
+    """The staged kernel (next plane's input streamed into LDS during the current plane) gives
+    the same bits as the kernel whose first pass reads global memory itself, with work-groups
+    walking several planes (grid smaller than the plane count)."""
+    cfg = _cfg(M, dims, K, 4, t)
+    shape = cfg.shape
+    D = len(dims)
+    half = list(shape)
+    half[1] = dims[0] // 2 + 1
+    ishape = half if t == 2 else shape
+    n_in = oracle.extent(D, ishape, cfg.istride)
+    x = oracle.uniform_real(9, n_in) if t == 1 else oracle.uniform_complex(9, n_in)
+    x = x.astype(np.float32 if t == 1 else np.complex64)
+    ys = []
+    for stage in ("on", "off"):
+        if stage == "off":
+            monkeypatch.setenv("BBFFT_FUSE_STAGE", "off")
+        fc = bb.fused_configuration(cfg)
+        src = bb.generate_fused_source(cfg)
+        assert ("stage_copy" in src) == (stage == "on")
+        oshape = half if t == 1 else shape
+        n_out = oracle.extent(D, oshape, cfg.ostride)
+        ys.append(_run_plan(cfg, x, n_out))
+    assert np.array_equal(ys[0], ys[1])
+
+
 def test_fused_in_place_layout_r2c():
     """In-place r2c layout (padded real rows) read by the fused kernel."""
     cfg = _cfg(1, [16, 8, 4], 2, 4, 1, inplace=True)
@@ -97,7 +126,13 @@ def test_fusion_policy(monkeypatch):
     monkeypatch.setenv("BBFFT_FUSE", "on")
     fc = bb.fused_configuration(c3)
     assert fc["first_pass"] == 0 and fc["planes"] == 128 * 32 and fc["plane_elems"] == 65 * 128
-    assert fc["lds_bytes"] <= 80 * 1024 and fc["group_size"] <= 1024
+    # plane + stage scratch <= 80 KB, plus the 64 KB staging buffer of the next plane's input
+    assert fc["lds_bytes"] <= 160 * 1024 and fc["group_size"] <= 1024
+    monkeypatch.setenv("BBFFT_FUSE_STAGE", "off")
+    unstaged = bb.fused_configuration(c3)
+    assert unstaged["lds_bytes"] <= 80 * 1024 and unstaged["name"] != fc["name"]
+    assert fc["lds_bytes"] == unstaged["lds_bytes"] + 4 * 128 * 128
+    monkeypatch.delenv("BBFFT_FUSE_STAGE")
     fr = bb.fused_configuration(c2r)
     assert fr["first_pass"] == 1 and fr["kb"] == 128 and fr["ka"] == 1
     assert bb.fused_configuration(bb.parse_fft_descriptor("scfo256x256*4")) is None

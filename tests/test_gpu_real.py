@@ -183,3 +183,32 @@ def test_fused_passes_forced(desc, monkeypatch):
     tol = oracle.tolerance(int(cfg.fp), int(np.prod(Ns)))
     assert oracle.rel_l2(y_f, ref) <= tol
     assert oracle.rel_l2(y_f, y_p) <= tol
+
+
+@pytest.mark.parametrize("desc", ["srfo128x128x128*4", "srbo128x128x128*4", "scfo64x64x64*8",
+                                  "scbo96x80*400", "drfo48x40x8*400"])
+def test_fused_staged_vs_unstaged(desc, monkeypatch):
+    """Fused kernels that stream the next plane's input into LDS (direct global-to-LDS loads,
+    persistent grid; more planes than resident work-groups, so groups walk several planes)
+    give the same bits as the fused kernels whose first pass reads global memory itself
+    (BBFFT_FUSE_STAGE=off)."""
+    cfg = bb.parse_fft_descriptor(desc)
+    monkeypatch.setenv("BBFFT_FUSE", "on")
+    ys = []
+    for stage in ("on", "off"):
+        if stage == "off":
+            monkeypatch.setenv("BBFFT_FUSE_STAGE", "off")
+        plan = bb.make_plan(cfg, cache=bb.default_cache())
+        fused = [l for l in plan.launches() if l["kernel"].startswith("bbfft_fused2")]
+        assert len(fused) == 1
+        if stage == "on":
+            fc = bb.fused_configuration(cfg)
+            assert fused[0]["grid"] < fc["planes"]
+        rng = np.random.default_rng(3)
+        ishape, _ = io_shapes(cfg)
+        n_in = oracle.extent(cfg.dim, ishape, cfg.istride)
+        t = int(cfg.type)
+        x = rng.uniform(-1, 1, n_in) if t == 1 else (rng.uniform(-1, 1, n_in) +
+                                                      1j * rng.uniform(-1, 1, n_in))
+        ys.append(execute(cfg, x, plan=plan))
+    assert np.array_equal(ys[0].view(np.uint8), ys[1].view(np.uint8))
