@@ -170,7 +170,12 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
                                      ? static_cast<std::size_t>(pa.fp)
                                      : 2 * static_cast<std::size_t>(pa.fp);
     std::size_t const words = (fc.in_plane_stride * in_bytes / 4 + 63) / 64 * 64;
-    if (!stage_off && info.type == device_type::gpu &&
+    // the plane's input must lie within its in_plane_stride elements (it does for the
+    // default packed and in-place layouts that multi-dimensional plans use)
+    std::size_t const na_in = pa.shape[1];
+    std::size_t const extent = (Ma - 1) * pa.istride[0] + (na_in - 1) * pa.istride[1] +
+                               (ka - 1) * pa.istride[2] + 1;
+    if (!stage_off && info.type == device_type::gpu && extent <= fc.in_plane_stride &&
         fc.lds_bytes() + 4 * words <= info.local_memory_size) {
         fused2_configuration st = fc;
         // pass a now reads LDS: no streaming hint on its loads
