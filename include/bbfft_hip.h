@@ -195,10 +195,13 @@ BBFFT_EXPORT_C int bbfft_device_info_query(int device, char *buf, size_t len);
 /*
  * Fused passes of 2-D/3-D plans (extension of the reference's nd_fft,
  * src/common/algorithm/nd_fft.hpp:52-99, which runs one memory round trip per mode): where a
- * plane of the intermediate tensor fits in LDS (and is small enough for several work-groups
- * per CU, or the problem is launch-bound), the passes over modes 1 and 2 run as one kernel.
- * These entry points report the decision and the kernel for a device_info (host only); plans
- * apply it automatically (BBFFT_FUSE=off disables it, BBFFT_FUSE=on forces it where it fits).
+ * plane of the intermediate tensor fits in LDS (up to 96 KB with at least 256 planes, or any
+ * size when the problem is launch-bound), the passes over modes 1 and 2 run as one kernel.
+ * Where a second buffer for one plane's input also fits, the kernel streams the next plane's
+ * input into it during the current plane (lds_bytes below includes that buffer). These entry
+ * points report the decision and the kernel for a device_info (host only); plans apply it
+ * automatically (BBFFT_FUSE=off disables fusion, BBFFT_FUSE=on forces it where it fits,
+ * BBFFT_FUSE_STAGE=off disables the input staging).
  */
 typedef struct bbfft_fused_info {
     unsigned first_pass;
