@@ -240,7 +240,8 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
     std::size_t const BS = std::min<std::size_t>(64, fc.group_size);
     std::size_t const NWc = fc.group_size >= 64 ? fc.group_size / 64 : 1;
     std::size_t const NB = (W + BS - 1) / BS;
-    os << "  __shared__ float stage[" << fc.stage_words << "];\n"
+    // 16-byte aligned: pass a may read the staged input as float2 / double2
+    os << "  __shared__ __attribute__((aligned(16))) float stage[" << fc.stage_words << "];\n"
        << "  const unsigned wv = threadIdx.x / 64u, ln = threadIdx.x % 64u;\n"
        << "  auto stage_copy = [&](unsigned long f) {\n"
        << "    const float* src = (const float*)in + f * " << W << "ul;\n"
