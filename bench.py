@@ -70,6 +70,36 @@ def time_plan(bb, torch, desc, steps, warmup, cache):
             "frac": nbytes / ms / 1e6 / HBM_PEAK_GBPS, "bytes": nbytes, "kernel": launch}
 
 
+def torch_fft_comparator(torch, desc, steps):
+    """The same transform through torch.fft (rocFFT / hipFFT on ROCm) on the same tensor
+    layout: a vendor-library reference point, not part of the library."""
+    import bbfft_amd as bb
+    cfg = bb.parse_fft_descriptor(desc)
+    M, N, K = cfg.shape[:3]
+    if cfg.dim != 1 or int(cfg.type) != 0:
+        return None
+    rdt = torch.float32 if int(cfg.fp) == 4 else torch.float64
+    x = torch.view_as_complex(torch.rand((K, N, M, 2), device="cuda", dtype=rdt) * 2 - 1)
+    x = x.contiguous()
+    out = torch.empty_like(x)
+    fn = (lambda: torch.fft.fft(x, dim=1, out=out)) if int(cfg.dir) < 0 else \
+        (lambda: torch.fft.ifft(x, dim=1, norm="forward", out=out))
+    for _ in range(5):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / steps
+    nbytes = 2.0 * x.numel() * x.element_size()
+    del x, out
+    return {"ms": round(ms, 5), "GBps": round(nbytes / ms / 1e6, 1),
+            "GFLOPs": round(5.0 * N * math.log2(N) * M * K / ms / 1e6, 1)}
+
+
 def stream_copy_bandwidth(torch, nbytes=1 << 30, reps=10):
     """Read + write GB/s of the hand-written gfx950 copy kernels (lib/libbbfft_stream.so):
     the practical streaming ceiling for a 1 GiB -> 1 GiB pass. None if the probe is not built."""
@@ -355,9 +385,10 @@ def main():
         for p in "sd":
             for n in (16, 32, 64, 128, 256, 512, 60, 105, 243, 384):
                 r = time_plan(bb, torch, f"{p}cfo8.{n}*32768", max(5, a.steps // 2), 2, cache)
+                v = torch_fft_comparator(torch, r["desc"], max(5, a.steps // 2))
                 sweep.append({"desc": r["desc"], "ms": round(r["ms"], 5),
                               "GFLOPs": round(r["gflops"], 1), "GBps": round(r["GBps"], 1),
-                              "frac": round(r["frac"], 4)})
+                              "frac": round(r["frac"], 4), "torch_fft_GBps": v["GBps"]})
                 print(json.dumps(sweep[-1]), file=sys.stderr, flush=True)
     nd = None
     if not a.no_nd:
@@ -377,6 +408,7 @@ def main():
         print(json.dumps({"ragged": rag}), file=sys.stderr, flush=True)
     copy_gbps = copy_bandwidth(torch)
     stream = stream_copy_bandwidth(torch)
+    vendor = torch_fft_comparator(torch, a.desc, min(a.steps, 100))
     cpu = None if a.no_cpu else cpu_baseline()
     cfg = bb.parse_fft_descriptor(a.desc)
     line = {
@@ -399,6 +431,7 @@ def main():
                    "frac_of_stream_copy": (round(head["GBps"] / max(stream.values()), 4)
                                            if stream else None),
                    "kernel": head["kernel"]["kernel"],
+                   "torch_fft_same_tensor": vendor,
                    "sweep": sweep, "nd_real_3d_32x128cubed": nd,
                    "autograd_3d_filter_32x128cubed": grad, "mfma_split_n512": mf,
                    "ragged_serving": rag},
