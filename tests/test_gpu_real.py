@@ -103,9 +103,15 @@ def test_r2c_c2r_identity(case):
     assert approx_equal(got, x[ioff], eps).all()
 
 
+# the tuned M=8 real-transform entries (persistent grids, load-first loops), odd K included
+_M8_REAL = [f"{p}r{d}o8.{n}*{k}" for p in "sd" for d in "fb"
+            for n, k in ((16, 301), (32, 77), (64, 40), (128, 33), (256, 17), (512, 9),
+                         (60, 45), (105, 21), (243, 13), (384, 11))]
+
+
 @pytest.mark.parametrize("desc", ["srfo3.16*5", "drfo3.27*6", "srfi1.64*7", "srfo2.5x6*3",
                                   "drfo1.8x6x5*2", "srbo3.16*5", "drbo3.27*6", "srbi1.64*7",
-                                  "srbo2.5x6*3", "drbo1.8x6x5*2", "srbo2.16*3"])
+                                  "srbo2.5x6*3", "drbo1.8x6x5*2", "srbo2.16*3"] + _M8_REAL)
 def test_real_random_vs_oracle(desc):
     """Seeded random inputs vs the FP64 oracle (rel-L2 <= 8 eps log2 N). c2r inputs have a
     non-zero imaginary part in the DC bin (ignored) and, for even N_1, in the Nyquist bin
