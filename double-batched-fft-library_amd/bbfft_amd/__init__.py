@@ -326,7 +326,10 @@ class Cache:
         return self._h
 
     def __del__(self):
-        lib = _capi._lib
+        try:
+            lib = _capi._lib
+        except (AttributeError, NameError):  # interpreter shutdown cleared the module
+            return
         if lib is not None and getattr(self, "_h", None) and self._h.value:
             lib.bbfft_cache_destroy(self._h)
             self._h = ctypes.c_void_p()
@@ -383,7 +386,10 @@ class Plan:
         return out
 
     def __del__(self):
-        lib = _capi._lib
+        try:
+            lib = _capi._lib
+        except (AttributeError, NameError):  # interpreter shutdown cleared the module
+            return
         if lib is not None and getattr(self, "_h", None):
             lib.bbfft_plan_destroy(self._h)
             self._h = None
