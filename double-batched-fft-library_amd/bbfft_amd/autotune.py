@@ -236,11 +236,13 @@ class Tuner:
             best, best_ms = self._try(best, best_ms, opts)
         return best, best_ms
 
-    def confirm(self, top=4, rounds=6):
+    def confirm(self, top=4, rounds=6, reps=None):
         """Re-decide between the `top` fastest distinct candidates of the search by interleaved
-        timing in this process: `rounds` rounds, each timing every candidate once in turn, so
-        slow drifts of the device (clocks, temperature) hit all of them alike. Returns the
-        candidate with the best median and that median (ms)."""
+        timing in this process: `rounds` rounds, each timing every candidate once in turn over
+        `reps` launches (default: the search's), so slow drifts of the device (clocks,
+        temperature) hit all of them alike; long runs time the kernels at the sustained clocks
+        of a long job. Returns the candidate with the best median and that median (ms)."""
+        reps = reps or self.reps
         import statistics
         torch = self.torch
         ranked = sorted((ms, k) for k, ms in self.results.items() if ms < float("inf"))[:top]
@@ -252,11 +254,11 @@ class Tuner:
                 plan.execute(self.x, self.y)
                 s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 s.record()
-                for _ in range(self.reps):
+                for _ in range(reps):
                     plan.execute(self.x, self.y)
                 e.record()
                 torch.cuda.synchronize()
-                times[i].append(s.elapsed_time(e) / self.reps)
+                times[i].append(s.elapsed_time(e) / reps)
         med = [statistics.median(t) for t in times]
         for (t, _), m in zip(plans, med):
             self.log(desc=self.desc, confirm=t, median_ms=round(m, 5),
@@ -329,6 +331,9 @@ def main(argv=None):
     ap.add_argument("--confirm", type=int, default=6, metavar="ROUNDS",
                     help="re-time the 4 fastest candidates interleaved over ROUNDS rounds and "
                          "keep the best median (0: keep the search's best)")
+    ap.add_argument("--confirm-reps", type=int, default=0, metavar="N",
+                    help="launches per candidate and round of the confirmation (default: "
+                         "--reps; about 150 times candidates at sustained clocks)")
     ap.add_argument("descriptors", nargs="+")
     a = ap.parse_args(argv)
     lines = []
@@ -340,7 +345,7 @@ def main(argv=None):
         tuner = Tuner(desc, reps=a.reps, variant=variant)
         best, best_ms = tuner.run(a.rounds)
         if a.confirm > 0:
-            best, best_ms = tuner.confirm(rounds=a.confirm)
+            best, best_ms = tuner.confirm(rounds=a.confirm, reps=a.confirm_reps or None)
         line = table_line(tuner.cfg, best, tuner.default_ms, best_ms, tuner.bytes, desc)
         lines.append(line)
         print(json.dumps({"desc": desc, "best": best, "ms": round(best_ms, 5),
