@@ -78,6 +78,8 @@ class nd_plan final : public plan_base {
     nd_plan(pass_layout const &layout, hipStream_t stream, int device, jit_cache *cache,
             unsigned real_pass_variant = 0u);
     ~nd_plan();
+    nd_plan(nd_plan const &) = delete; // owns the scratch buffer
+    nd_plan &operator=(nd_plan const &) = delete;
     void execute(void const *in, void *out, hipEvent_t signal, std::uint32_t nwait,
                  hipEvent_t *wait) override;
     std::vector<launch_info> launches() const override;

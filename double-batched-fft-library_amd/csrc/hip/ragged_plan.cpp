@@ -72,8 +72,15 @@ ragged_plan::ragged_plan(ragged_configuration const &cfg, hipStream_t stream, in
     auto table = ragged_job_table(cfg, rk_, grid_);
     njobs_ = cfg.jobs.size();
     BBFFT_HIP_CHECK(hipMalloc(&table_, table.size() * sizeof(std::uint64_t)));
-    BBFFT_HIP_CHECK(hipMemcpy(table_, table.data(), table.size() * sizeof(std::uint64_t),
-                              hipMemcpyHostToDevice));
+    hipError_t const copied = hipMemcpy(table_, table.data(),
+                                        table.size() * sizeof(std::uint64_t),
+                                        hipMemcpyHostToDevice);
+    if (copied != hipSuccess) {
+        // a throwing constructor runs no destructor: release the table here
+        (void)hipFree(table_);
+        table_ = nullptr;
+        BBFFT_HIP_CHECK(copied);
+    }
 }
 
 ragged_plan::~ragged_plan() {
