@@ -355,16 +355,60 @@ def _current_stream():
     return 0
 
 
+def _io_bytes(cfg, in_elem_bytes=None):
+    """Bytes from the first element to the last one a configuration addresses, (input,
+    output)."""
+    rb = int(cfg.fp)
+    t = int(cfg.type)
+    ishape, oshape = list(cfg.shape), list(cfg.shape)
+    if t == 2:
+        ishape[1] = cfg.shape[1] // 2 + 1
+    if t == 1:
+        oshape[1] = cfg.shape[1] // 2 + 1
+
+    def extent(shape, stride):
+        return sum((shape[i] - 1) * stride[i] for i in range(cfg.dim + 2)) + 1
+
+    ieb = in_elem_bytes or (rb if t == 1 else 2 * rb)
+    oeb = rb if t == 2 else 2 * rb
+    return extent(ishape, cfg.istride) * ieb, extent(oshape, cfg.ostride) * oeb
+
+
+def _check_buffer(x, need, what):
+    """A tensor argument must live on the GPU and hold `need` bytes from its data pointer on:
+    the kernels address the whole configured tensor, and a smaller buffer would be read or
+    written out of bounds. Plain integer pointers are passed through unchecked."""
+    if not hasattr(x, "untyped_storage"):
+        return
+    if not x.is_cuda:
+        raise ValueError(f"{what} must be a GPU tensor (got device {x.device})")
+    avail = x.untyped_storage().nbytes() - x.storage_offset() * x.element_size()
+    if avail < need:
+        raise ValueError(f"{what} holds {avail} bytes from its data pointer, the configuration "
+                         f"addresses {need}")
+
+
 class Plan:
     """Mirror of plan_unmanaged_event<hipEvent_t> (include/bbfft/plan.hpp:144-184)."""
 
-    def __init__(self, handle, cfg, stream):
+    def __init__(self, handle, cfg, stream, in_elem_bytes=None):
         self._h = handle
         self.cfg = cfg
+        # bytes the input and output span (None: not checked, callbacks interpret offsets)
+        self._io = _io_bytes(cfg, in_elem_bytes) if cfg is not None and not cfg.callbacks \
+            else None
         self.stream = stream
 
     def execute(self, inp, out=None, signal=None, wait: Sequence = ()):
-        """Out-of-place if out is given, else in-place. Events are hipEvent_t handles (int)."""
+        """Out-of-place if out is given, else in-place. Events are hipEvent_t handles (int).
+        Tensors are checked for device and size (see _check_buffer); integer pointers are not."""
+        if self._io is not None:
+            ib, ob = self._io
+            if out is None:
+                _check_buffer(inp, max(ib, ob), "the in-place buffer")
+            else:
+                _check_buffer(inp, ib, "the input")
+                _check_buffer(out, ob, "the output")
         src = _ptr(inp)
         dst = src if out is None else _ptr(out)
         nw = len(wait)
@@ -486,7 +530,7 @@ def make_half_input_plan(cfg, storage="f16", mfma=True, stream=None,
     _check(_capi.load().bbfft_plan_create_half_input(
         ctypes.byref(cfg._to_c()), half_input_flags(storage, mfma), stream,
         cache.handle if cache is not None else None, ctypes.byref(h)))
-    return Plan(h, cfg, stream)
+    return Plan(h, cfg, stream, in_elem_bytes=4)
 
 
 DEFAULT_AOT_DIR = os.path.join(os.path.dirname(_capi.DEFAULT_LIB), "aot")

@@ -147,3 +147,23 @@ def test_launch_introspection():
     assert l["kernel"] == kc["name"] and l["kernel"].startswith("bbfft_m1_M8_Mb8_N256")
     assert l["block"] == kc["work_group_size"]
     assert l["grid"] == 32768 // kc["Kb"]
+
+
+def test_execute_rejects_small_device_buffers():
+    """Tensor arguments smaller than the configured tensor are refused before any launch (the
+    kernel would address them out of bounds); exact-size tensors and views into larger
+    storage run."""
+    import torch
+    cfg = bb.parse_fft_descriptor("scfo8.256*64")
+    plan = bb.make_plan(cfg, cache=bb.default_cache())
+    n = 8 * 256 * 64
+    x = torch.zeros(n, dtype=torch.complex64, device="cuda")
+    y = torch.zeros(n, dtype=torch.complex64, device="cuda")
+    small = torch.zeros(n - 1, dtype=torch.complex64, device="cuda")
+    for args in ((small, y), (x, small), (small,)):
+        with pytest.raises(ValueError, match="holds"):
+            plan.execute(*args)
+    big = torch.zeros(n + 16, dtype=torch.complex64, device="cuda")
+    plan.execute(x, big[16:])
+    plan.execute(big[:n])
+    torch.cuda.synchronize()

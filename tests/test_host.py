@@ -338,3 +338,25 @@ def test_c_abi_rejects_invalid_arguments():
     assert L.bbfft_configuration_to_string(ctypes.byref(c), None, 0) == \
         _capi.ERROR_INVALID_ARGUMENT
     assert L.bbfft_plan_execute(None, None, None, None, 0, None) == _capi.ERROR_INVALID_ARGUMENT
+
+
+def test_plan_execute_checks_tensor_arguments():
+    """Plan.execute rejects host tensors and device buffers smaller than the configured tensor
+    before anything reaches the library (a smaller buffer would be addressed out of bounds);
+    the byte counts follow the configuration's extents: scfo8.256*4 spans 8 x 256 x 4 complex
+    elements, srfo256*4 reads 256 x 4 reals and writes 129 x 4 complex elements."""
+    import ctypes
+    import torch
+    cfg = bb.parse_fft_descriptor("scfo8.256*4")
+    assert bb._io_bytes(cfg) == (8 * 256 * 4 * 8, 8 * 256 * 4 * 8)
+    r2c = bb.parse_fft_descriptor("srfo256*4")
+    assert bb._io_bytes(r2c) == (256 * 4 * 4, 129 * 4 * 8)
+    c2r = bb.parse_fft_descriptor("drbo256*4")
+    assert bb._io_bytes(c2r) == (129 * 4 * 16, 256 * 4 * 8)
+    assert bb._io_bytes(cfg, in_elem_bytes=4)[0] == 8 * 256 * 4 * 4
+    plan = bb.Plan(ctypes.c_void_p(), cfg, 0)
+    x = torch.empty(8 * 256 * 4, dtype=torch.complex64)
+    with pytest.raises(ValueError, match="GPU tensor"):
+        plan.execute(x, x)
+    with pytest.raises(ValueError, match="GPU tensor"):
+        plan.execute(x)
