@@ -14,8 +14,9 @@ pytestmark = [pytest.mark.gpu,
 def test_autotune_writes_confirmed_entry(tmp_path, capsys):
     from bbfft_amd import autotune
     out = tmp_path / "table.inc"
-    assert autotune.main(["--rounds", "1", "--reps", "3", "--confirm", "2", "-o", str(out),
-                          "scfo8.64*4096"]) == 0
+    # the confirmation runs longer (sustained) rounds than the search
+    assert autotune.main(["--rounds", "1", "--reps", "3", "--confirm", "2",
+                          "--confirm-reps", "12", "-o", str(out), "scfo8.64*4096"]) == 0
     lines = [ln for ln in out.read_text().splitlines() if autotune._key(ln)]
     assert len(lines) == 1 and autotune._key(lines[0]) == ("f32", "c2c", "64", "8")
     logged = [json.loads(ln) for ln in capsys.readouterr().out.splitlines() if ln.startswith("{")]
