@@ -3,6 +3,7 @@
 #include "bbfft/parser.hpp"
 
 #include <cctype>
+#include <limits>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -37,8 +38,14 @@ class cursor {
     std::size_t number() {
         std::size_t start = pos_;
         std::size_t v = 0;
+        // numbers are ints, as the reference reads them (std::stoi, src/base/parser.cpp:48)
+        std::size_t const max = static_cast<std::size_t>(std::numeric_limits<int>::max());
         while (!at_end() && std::isdigit(static_cast<unsigned char>(text_[pos_]))) {
-            v = 10 * v + static_cast<std::size_t>(text_[pos_] - '0');
+            std::size_t const d = static_cast<std::size_t>(text_[pos_] - '0');
+            if (v > (max - d) / 10) {
+                fail("number out of range");
+            }
+            v = 10 * v + d;
             ++pos_;
         }
         if (start == pos_) {

@@ -68,6 +68,16 @@ def test_malformed_descriptors_throw(bad):
     assert "is malformed" in str(e.value)
 
 
+@pytest.mark.parametrize("bad", ["scfo2147483648*1", "scfo8.256*99999999999999999999",
+                                 "scfo18446744073709551617*2", "scfi16*32i1,4294967296"])
+def test_descriptor_numbers_out_of_int_range_throw(bad):
+    """Descriptor numbers are ints, as the reference reads them (std::stoi throws beyond
+    2^31 - 1); nothing wraps around."""
+    with pytest.raises(bb.DescriptorError, match="out of range"):
+        bb.parse_fft_descriptor(bad)
+    assert bb.parse_fft_descriptor("scfo2147483647*1").shape[1] == 2147483647
+
+
 def test_malformed_descriptor_caret_points_at_error():
     with pytest.raises(bb.DescriptorError) as e:
         bb.parse_fft_descriptor("srb4x5*6x7")
