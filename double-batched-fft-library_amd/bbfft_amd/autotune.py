@@ -331,9 +331,9 @@ def main(argv=None):
     ap.add_argument("--confirm", type=int, default=6, metavar="ROUNDS",
                     help="re-time the 4 fastest candidates interleaved over ROUNDS rounds and "
                          "keep the best median (0: keep the search's best)")
-    ap.add_argument("--confirm-reps", type=int, default=0, metavar="N",
-                    help="launches per candidate and round of the confirmation (default: "
-                         "--reps; about 150 times candidates at sustained clocks)")
+    ap.add_argument("--confirm-reps", type=int, default=100, metavar="N",
+                    help="launches per candidate and round of the confirmation; long runs rank "
+                         "the candidates at the sustained clocks of a long job (0: --reps)")
     ap.add_argument("descriptors", nargs="+")
     a = ap.parse_args(argv)
     lines = []
