@@ -12,8 +12,10 @@ candidate's output is compared with the default plan's output on the same input 
 within the precision's tolerance) before it is timed; the default plan itself is covered by the
 GPU parity suite against the FP64 oracle. Time = best of 3 runs of `reps` launches (HIP events).
 Confirmation (`--confirm ROUNDS`, default 6): the 4 fastest candidates are re-timed in turn,
-round after round, and the best median wins, so that a few per cent of drift between the
-search's measurements does not decide close calls.
+round after round, over `--confirm-reps` launches each (default 100), and the best median
+wins, so that a few per cent of drift between the search's measurements does not decide close
+calls and the candidates are ranked at sustained clocks (short bursts can rank LDS-bound
+kernels differently).
 """
 import argparse
 import itertools
