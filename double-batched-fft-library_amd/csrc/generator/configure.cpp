@@ -64,10 +64,29 @@ std::string fft1d_configuration::identifier() const {
     if (store_function) {
         oss << '_' << store_function;
     }
+    if (callback_hash) {
+        oss << "_cb" << std::hex << callback_hash;
+    }
     return oss.str();
 }
 
 namespace gen {
+
+std::uint64_t callback_hash(user_module const &um) {
+    if (!um) {
+        return 0;
+    }
+    std::uint64_t h = 0xcbf29ce484222325ull;
+    auto mix = [&h](unsigned char b) {
+        h ^= b;
+        h *= 0x100000001b3ull;
+    };
+    for (std::size_t i = 0; i < um.length; ++i) {
+        mix(static_cast<unsigned char>(um.data[i]));
+    }
+    mix(static_cast<unsigned char>(um.language));
+    return h ? h : 1;
+}
 
 /// Stage radices for a complex FFT of length n: one stage up to `single_max`, otherwise the
 /// fewest stages whose largest radix does not exceed `radix_max` (factors closest to n^(1/L)).
@@ -125,6 +144,11 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     bool const mfma = (all_flags & fft1d_mfma) != 0;
     if (!explicit_tuning && info.type == device_type::gpu && info.max_subgroup_size() == 64) {
         gen::lookup_tuning(cfg.fp, cfg.type, N, M, all_flags, tuning);
+        // The table is measured on plain kernels; the load-first tile loop needs direct global
+        // loads, so a plan with callbacks keeps the entry's tiling without it.
+        if (cfg.callbacks) {
+            tuning.flags &= ~static_cast<unsigned>(fft1d_prefetch);
+        }
     }
 
     fft1d_configuration c = {};
@@ -137,6 +161,7 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     c.ostride = {cfg.ostride[0], cfg.ostride[1], cfg.ostride[2]};
     c.load_function = cfg.callbacks.load_function;
     c.store_function = cfg.callbacks.store_function;
+    c.callback_hash = gen::callback_hash(cfg.callbacks);
     std::size_t const nfft = c.N_fft();
 
     if (half_in) {

@@ -48,6 +48,9 @@ struct BBFFT_EXPORT fft1d_configuration {
     bool inplace_unsupported;            ///< in == out would race across work-groups
     char const *load_function;           ///< user load callback or nullptr
     char const *store_function;          ///< user store callback or nullptr
+    /// FNV-1a hash of the callback source and its language (0 without callbacks): two plans
+    /// whose callbacks share names but not code get different identifiers and cache entries.
+    std::uint64_t callback_hash = 0;
 
     /// Kernel symbol name; also the jit_cache key.
     std::string identifier() const;

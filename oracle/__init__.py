@@ -13,8 +13,10 @@ Contents
 
 Pinning: tests/test_oracle.py checks ``fft`` against numpy's pocketfft, the reference's
 analytic known answers (test/c2c.cpp:28-47, test/r2c.cpp:40-131), Parseval, linearity and the
-forward/backward identity. The c2r Nyquist rule has no external pin beyond the derivation
-cited in dft_oracle.c ("parity unpinned" for inputs with a non-zero Nyquist imaginary part).
+forward/backward identity. The c2r handling of imag(X[0]) and imag(X[N/2]) on arbitrary
+(non-Hermitian) half spectra is pinned by an independent numpy restatement of the reference's
+half-length pre-pass and odd-length column pairing (src/base/generator/sbfft_gen.cpp:226-247,
+318-351), in 1-D and in the nd order of src/common/algorithm/nd_fft.hpp:89-94.
 """
 import ctypes
 import math

@@ -68,3 +68,60 @@ def test_store_callback_truncate_scale(fp, M, N, lang):
     expect_im = (X_ref.imag[:, :n_cut, :].astype(real_np) * real_np(scale))
     assert np.array_equal(X.real, expect_re)
     assert np.array_equal(X.imag, expect_im)
+
+
+@pytest.mark.parametrize("desc", ["scfo8.256*4096", "srfo8.256*4096"])
+def test_callback_plan_at_tuned_shape_matches_plain_plan(desc):
+    """A tuned shape whose table entry carries the load-first tile loop, with an identity load
+    callback: bitwise equal to the plain plan."""
+    cfg_ref = bb.parse_fft_descriptor(desc)
+    cfg = bb.parse_fft_descriptor(desc)
+    t = "float" if cfg.type == bb.transform_type.r2c else "float2"
+    cfg.callbacks = bb.user_module(f"__device__ {t} ld({t}* p, size_t o) {{ return p[o]; }}",
+                                   "ld")
+    M, N, K = cfg.shape[:3]
+    if cfg.type == bb.transform_type.r2c:
+        x = oracle.uniform_real(23, M * N * K)
+    else:
+        x = oracle.uniform_complex(23, M * N * K)
+    assert np.array_equal(execute(cfg, x), execute(cfg_ref, x))
+
+
+def test_same_callback_name_different_code_in_one_cache():
+    """Two store callbacks named alike with different scale constants, built through one
+    cache: each plan runs its own code."""
+    M, N, K = 1, 64, 16
+    x = oracle.uniform_real(24, M * N * K)
+    cfg_ref = bb.parse_fft_descriptor(f"srfo{M}.{N}*{K}")
+    X = execute(cfg_ref, x)
+    cache = bb.Cache()
+    outs = []
+    for scale in ("2.0f", "3.0f"):
+        cfg = bb.parse_fft_descriptor(f"srfo{M}.{N}*{K}")
+        cfg.callbacks = bb.user_module(
+            "__device__ void store(float2* o, size_t i, float2 v) "
+            f"{{ v.x *= {scale}; v.y *= {scale}; o[i] = v; }}", None, "store")
+        outs.append(execute(cfg, x, plan=bb.make_plan(cfg, cache=cache)))
+    for out, scale in zip(outs, (np.float32(2), np.float32(3))):
+        assert np.array_equal(out.real, X.real * scale)
+        assert np.array_equal(out.imag, X.imag * scale)
+
+
+@pytest.mark.parametrize("fp,M,N", [(4, 1, 64), (4, 32, 212), (8, 1, 212), (8, 32, 8)])
+def test_store_callback_truncate_scale_against_oracle(fp, M, N):
+    """The truncate-and-scale store callback against the FP64 oracle's r2c times 1/N."""
+    K = 16
+    n_ref, n_cut = N // 2 + 1, N // 4
+    istride = [1, M, M * N]
+    ostride = [1, M, M * n_ref]
+    real_t = _real_t(fp)
+    scale = float(np.float32(1.0 / N)) if fp == 4 else 1.0 / N
+    cfg = bb.configuration(1, [M, N, K], bb.precision(fp), bb.direction.forward,
+                           bb.transform_type.r2c, istride, ostride,
+                           bb.user_module(cbs.store_hip(real_t, M, n_ref, n_cut, scale.hex()),
+                                          None, "store"))
+    x = oracle.uniform_real(25, M * N * K)
+    xr = x.astype(np.float32).astype(np.float64) if fp == 4 else x
+    ref = oracle.fft(1, [M, N, K], istride, ostride, 1, -1, xr).reshape(K, n_ref, M)
+    got = execute(cfg, x)[:M * n_cut * K].reshape(K, n_cut, M)
+    assert oracle.rel_l2(got, ref[:, :n_cut, :] * scale) <= oracle.tolerance(fp, N)

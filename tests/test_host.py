@@ -294,6 +294,40 @@ def test_tuning_table_is_applied():
             assert f"_g{flags}" in kc["name"], desc
 
 
+@pytest.mark.parametrize("desc", ["scfo8.256*32768", "srfo8.256*32768", "scfo8.16*32768",
+                                  "scfo8.512*32768", "dcfo8.60*32768"])
+def test_callback_plans_take_table_tiling_without_prefetch(desc):
+    """Tuned shapes whose table entry carries the load-first tile loop (flag 64) still accept
+    callbacks: the plan keeps the entry's tiling and drops the flag."""
+    cfg = bb.parse_fft_descriptor(desc)
+    plain = bb.kernel_configuration(cfg)
+    fp = "float" if cfg.fp == bb.precision.f32 else "double"
+    t = fp if cfg.type == bb.transform_type.r2c else fp + "2"
+    cfg.callbacks = bb.user_module(
+        f"__device__ {t} ld({t}* p, size_t o) {{ return p[o]; }}", "ld")
+    kc = bb.kernel_configuration(cfg)
+    assert (kc["factorization"], kc["Nt"], kc["Mb"], kc["Kb"]) == \
+        (plain["factorization"], plain["Nt"], plain["Mb"], plain["Kb"])
+    flags = int(kc["name"].split("_g")[1].split("_")[0]) if "_g" in kc["name"] else 0
+    assert not flags & 64
+    assert "_cb" in kc["name"] and "_cb" not in plain["name"]
+
+
+def test_callback_source_and_language_enter_the_identifier():
+    """Same callback name, different code or language: different kernel identifiers (the
+    identifier is the jit_cache key and the symbol name)."""
+    def name(src, lang):
+        cfg = bb.parse_fft_descriptor("srfo1.64*16")
+        cfg.callbacks = bb.user_module(src, None, "store", lang)
+        return bb.kernel_configuration(cfg)["name"]
+    a = "__device__ void store(float2* o, size_t i, float2 v) { o[i] = v; }"
+    b = "__device__ void store(float2* o, size_t i, float2 v) { v.x *= 2.0f; o[i] = v; }"
+    names = {name(a, bb.kernel_language.hip), name(b, bb.kernel_language.hip),
+             name(a, bb.kernel_language.opencl_c)}
+    assert len(names) == 3
+    assert name(a, bb.kernel_language.hip) == name(a, bb.kernel_language.hip)
+
+
 def test_nd_decomposition_passes():
     """2-D/3-D plans run one double-batched pass per mode (reference nd_fft.hpp:61-99)."""
     cfg = bb.parse_fft_descriptor("srfo128x128x128*32")

@@ -114,8 +114,8 @@ def test_cosine_known_answer_r2c_and_c2r():
 
 def test_c2r_ignores_dc_imag_and_applies_nyquist_rule():
     """imag(X[0]) is ignored (reference test/r2c.cpp:310-324); for even N the reference's
-    half-length pre-pass adds -imag(X[N/2]) to every sample (derived from
-    src/base/generator/sbfft_gen.cpp:225-246; no reference test pins it)."""
+    half-length pre-pass adds -imag(X[N/2]) to every sample (src/base/generator/sbfft_gen.cpp:
+    225-246; pinned independently by the restatement tests below)."""
     for N in (7, 8):
         H = N // 2 + 1
         X = oracle.uniform_complex(8, H)
@@ -128,6 +128,129 @@ def test_c2r_ignores_dc_imag_and_applies_nyquist_rule():
             Xn[N // 2] += 0.75j
             diff = oracle.fft(1, [1, N, 1], [1, 1, N], [1, 1, N], 2, 1, Xn) - base
             assert np.allclose(diff, -0.75, atol=1e-14)
+
+
+def _ref_c2r_even(X, N):
+    """numpy restatement of the reference's even-length c2r, independent of the oracle:
+    the half-length pre-pass of src/base/generator/sbfft_gen.cpp:226-247 (f2fft_gen.cpp:350-366
+    is the same per i) for i = 0..N/4 -- imag(X[0]) zeroed, x1 = X[i], x2 = conj(X[N/2 - i]),
+    a = x1 + x2, b = (x1 - x2) * i w_N^i, z[i mod N/2] = a + b and, when the slots differ,
+    z[(N/2 - i) mod N/2] = (a.re - b.re, b.im - a.im) -- then the unnormalised N/2-point backward
+    FFT (numpy's pocketfft) and the de-interleave x[2n] = re z[n], x[2n+1] = im z[n]
+    (sbfft_gen.cpp:201-212, reshaped_mode(0, {2, N_fft}))."""
+    H = N // 2
+    z = np.zeros(H, complex)
+    for i in range(N // 4 + 1):
+        io = H - i
+        x1 = X[i]
+        if i == 0:
+            x1 = complex(x1.real, 0.0)
+        x2 = np.conj(X[io])
+        a = x1 + x2
+        b = (x1 - x2) * (np.exp(2j * np.pi * i / N) * 1j)
+        z[i % H] = a + b
+        if i % H != io % H:
+            z[io % H] = complex(a.real - b.real, b.imag - a.imag)
+    y = np.fft.ifft(z) * H
+    x = np.empty(N)
+    x[0::2] = y.real
+    x[1::2] = y.imag
+    return x
+
+
+def _ref_c2r_odd_pair(A, B, N):
+    """numpy restatement of the reference's odd-length c2r, which packs two K columns into one
+    complex FFT (src/base/generator/sbfft_gen.cpp:318-351): z[0] = (re A[0], re B[0]),
+    z[i] = A[i] + i B[i] and z[N - i] = conj(A[i]) + i conj(B[i]) for i = 1..N/2, then the
+    unnormalised N-point backward FFT; column a is the real part, column b the imaginary part."""
+    z = np.zeros(N, complex)
+    z[0] = complex(A[0].real, B[0].real)
+    for i in range(1, N // 2 + 1):
+        z[i] = A[i] + 1j * B[i]
+        z[N - i] = np.conj(A[i]) + 1j * np.conj(B[i])
+    y = np.fft.ifft(z) * N
+    return y.real, y.imag
+
+
+@pytest.mark.parametrize("N", [2, 4, 6, 8, 16, 48, 96, 128, 256, 300, 512])
+def test_c2r_even_matches_reference_prepass_on_non_hermitian_spectra(N):
+    """Fully random half spectra (non-zero imag(X[0]) and imag(X[N/2])): the oracle's c2r agrees
+    with the restated reference pre-pass. N/2 is odd for N = 2 and 6 (the i = N/4 slot is not
+    self-paired) and even for the other lengths."""
+    M, K = 3, 4
+    H = N // 2 + 1
+    X = oracle.uniform_complex(100 + N, M * H * K)
+    got = oracle.fft(1, [M, N, K], _packed([M, H, K]), _packed([M, N, K]), 2, 1, X)
+    Xa = X.reshape(K, H, M)
+    ref = np.empty((K, N, M))
+    for k in range(K):
+        for m in range(M):
+            ref[k, :, m] = _ref_c2r_even(Xa[k, :, m], N)
+    assert oracle.rel_l2(got, ref.ravel()) <= 1e-13
+
+
+@pytest.mark.parametrize("N", [1, 3, 5, 7, 15, 63, 105, 243])
+def test_c2r_odd_matches_reference_pair_packing(N):
+    M, K = 2, 6
+    H = N // 2 + 1
+    X = oracle.uniform_complex(200 + N, M * H * K)
+    got = oracle.fft(1, [M, N, K], _packed([M, H, K]), _packed([M, N, K]), 2, 1, X)
+    Xa = X.reshape(K, H, M)
+    ref = np.empty((K, N, M))
+    for k in range(0, K, 2):
+        for m in range(M):
+            ref[k, :, m], ref[k + 1, :, m] = _ref_c2r_odd_pair(Xa[k, :, m], Xa[k + 1, :, m], N)
+    assert oracle.rel_l2(got, ref.ravel()) <= 1e-13
+
+
+@pytest.mark.parametrize("dims", [[8, 6], [6, 5, 4], [16, 3], [7, 4, 3], [12, 10, 2]])
+def test_c2r_nd_matches_reference_order_on_non_hermitian_spectra(dims):
+    """Multi-dimensional c2r in the reference's order (src/common/algorithm/nd_fft.hpp:89-94):
+    complex backward transforms over modes 2..d (numpy), then the real transform over mode 1 as
+    restated above, on fully random half spectra."""
+    M, K = 2, 2
+    N1 = dims[0]
+    H = N1 // 2 + 1
+    half = [M, H] + dims[1:] + [K]
+    shape = [M] + dims + [K]
+    D = len(dims)
+    X = oracle.uniform_complex(300 + N1, int(np.prod(half)))
+    got = oracle.fft(D, shape, _packed(half), _packed(shape), 2, 1, X)
+    # numpy axes are the modes reversed: axis 0 = K, axis D + 1 = M, mode d = axis D + 1 - d
+    Xa = X.reshape(half[::-1])
+    for d in range(2, D + 1):
+        ax = D + 1 - d
+        Xa = np.fft.ifft(Xa, axis=ax) * Xa.shape[ax]
+    Xa = np.moveaxis(Xa, D, -1)  # mode 1 last
+    out = np.empty(Xa.shape[:-1] + (N1,))
+    for idx in np.ndindex(*Xa.shape[:-1]):
+        if N1 % 2 == 0:
+            out[idx] = _ref_c2r_even(Xa[idx], N1)
+    if N1 % 2 == 1:
+        # pair columns along K (axis 0), as the reference packs consecutive K columns
+        for idx in np.ndindex(*Xa.shape[1:-1]):
+            for k in range(0, K, 2):
+                out[(k,) + idx], out[(k + 1,) + idx] = _ref_c2r_odd_pair(
+                    Xa[(k,) + idx], Xa[(k + 1,) + idx], N1)
+    ref = np.moveaxis(out, -1, D)
+    assert oracle.rel_l2(got, ref.ravel()) <= 1e-13
+
+
+def test_c2r_restatement_detects_planted_nyquist_mistakes():
+    """The restatement separates the oracle's rule from the two plausible mistakes: dropping
+    imag(X[N/2]) or flipping its sign (both planted on the CPU, in a copy of the input)."""
+    N = 16
+    H = N // 2 + 1
+    X = oracle.uniform_complex(41, H)
+    ref = _ref_c2r_even(X, N)
+    dropped = X.copy()
+    dropped[N // 2] = dropped[N // 2].real
+    flipped = X.copy()
+    flipped[N // 2] = np.conj(flipped[N // 2])
+    assert oracle.rel_l2(_ref_c2r_even(dropped, N), ref) > 1e-3
+    assert oracle.rel_l2(_ref_c2r_even(flipped, N), ref) > 1e-3
+    got = oracle.fft(1, [1, N, 1], [1, 1, N], [1, 1, N], 2, 1, X)
+    assert oracle.rel_l2(got, ref) <= 1e-13
 
 
 def test_parseval_and_linearity():
