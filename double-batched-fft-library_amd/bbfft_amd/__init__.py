@@ -391,12 +391,13 @@ def _check_buffer(x, need, what):
 class Plan:
     """Mirror of plan_unmanaged_event<hipEvent_t> (include/bbfft/plan.hpp:144-184)."""
 
-    def __init__(self, handle, cfg, stream, in_elem_bytes=None):
+    def __init__(self, handle, cfg, stream, in_elem_bytes=None, io_bytes=None):
         self._h = handle
         self.cfg = cfg
-        # bytes the input and output span (None: not checked, callbacks interpret offsets)
+        # bytes the input and output span (None: not checked, callbacks interpret offsets);
+        # io_bytes gives them directly for plans without a configuration (ragged batches)
         self._io = _io_bytes(cfg, in_elem_bytes) if cfg is not None and not cfg.callbacks \
-            else None
+            else io_bytes
         self.stream = stream
 
     def execute(self, inp, out=None, signal=None, wait: Sequence = ()):

@@ -150,4 +150,14 @@ def make_ragged_plan(jobs, precision=4, stream=None, cache: Cache = None, graph=
     ch = cache.handle if cache is not None else None
     _check(_capi.load().bbfft_ragged_plan_create(precision, _c_jobs(jobs), len(jobs), flags,
                                                  stream, ch, ctypes.byref(h)))
-    return Plan(h, None, stream)
+    return Plan(h, None, stream, io_bytes=io_bytes(jobs, precision))
+
+
+def io_bytes(jobs: Sequence[Job], precision=4):
+    """(input, output) bytes a ragged launch addresses from the buffers' start: every job
+    reads [in_offset, in_offset + M N K) and writes [out_offset, out_offset + M N K) complex
+    elements."""
+    eb = 2 * precision
+    in_end = max((j.in_offset + j.elements for j in jobs), default=0)
+    out_end = max((j.out_offset + j.elements for j in jobs), default=0)
+    return in_end * eb, out_end * eb

@@ -28,7 +28,27 @@ typedef struct {
     long key[MAXPAIRS][4];
     FN(cv) v1[MAXPAIRS];
     FN(cv) v2[MAXPAIRS];
+    /* power_of_w(i, pwN) for i = 0..pwN-1: every root the DFTs of one plan use has a
+     * denominator dividing pwN and power_of_w depends only on the reduced fraction, so this
+     * table returns exactly power_of_w(k, n) (the reference bakes these values into the kernel
+     * source as literals; the port looks them up instead of re-evaluating them per element) */
+    const double *pw;
+    long pwN;
 } FN(pair_cache);
+
+static void FN(power_of_w_t)(FN(pair_cache) const *pc, long k, long n, double *re, double *im) {
+    if (pc->pw && n > 0 && pc->pwN % n == 0) {
+        long r = k % n;
+        if (r < 0) {
+            r += n;
+        }
+        long i = r * (pc->pwN / n);
+        *re = pc->pw[2 * i];
+        *im = pc->pw[2 * i + 1];
+        return;
+    }
+    cpu_power_of_w(k, n, re, im);
+}
 
 static void FN(simplify)(long k, long n, long *ko, long *no) {
     k %= n;
@@ -102,7 +122,7 @@ static void FN(esum)(FN(cv) * out, FN(cv) * const *x, int Nf, int dir, int kf, i
     }
     for (int s = 0; s < ns; ++s) {
         double wr, wi;
-        cpu_power_of_w(sk[s], sn[s] ? sn[s] : 1, &wr, &wi);
+        FN(power_of_w_t)(pc, sk[s], sn[s] ? sn[s] : 1, &wr, &wi);
         T r = (T)wr, i = (T)wi;
         for (int l = 0; l < nl; ++l) {
             T a = x[sj[s]]->re[l], b = x[sj[s]]->im[l];
@@ -137,7 +157,7 @@ static void FN(esum)(FN(cv) * out, FN(cv) * const *x, int Nf, int dir, int kf, i
             j2 = t;
         }
         double wr, wi;
-        cpu_power_of_w(k1, n1 ? n1 : 1, &wr, &wi);
+        FN(power_of_w_t)(pc, k1, n1 ? n1 : 1, &wr, &wi);
         T r = (T)wr, i = (T)wi;
         int c = pc->n++;
         pc->key[c][0] = pk1[p];
@@ -180,7 +200,7 @@ static void FN(inplace)(FN(cv) * x, int nl, const int *fac, int L, int dir, cons
                 for (int kf = 0; kf < Nf; ++kf) {
                     FN(esum)(&y[kf], xp, Nf, dir, kf, nl, pc);
                     double wr, wi;
-                    cpu_power_of_w((long)dir * kf * j, (long)J * Nf, &wr, &wi);
+                    FN(power_of_w_t)(pc, (long)dir * kf * j, (long)J * Nf, &wr, &wi);
                     T r = (T)wr, i = (T)wi;
                     if (tw && f == 0) {
                         int idx = cpu_scramble(fac, L, j + J * kf + J * Nf * k, 0);
@@ -387,6 +407,12 @@ static void *FN(worker)(void *arg) {
     FN(cv) *X1 = (FN(cv) *)malloc(sizeof(FN(cv)) * (size_t)p->N);
     FN(cv) *x = (FN(cv) *)malloc(sizeof(FN(cv)) * (size_t)(p->N > 64 ? p->N : 64));
     FN(pair_cache) *pc = (FN(pair_cache) *)malloc(sizeof(FN(pair_cache)));
+    double *pw = (double *)malloc(sizeof(double) * 2 * (size_t)p->N);
+    for (long i = 0; i < p->N; ++i) {
+        cpu_power_of_w(i, p->N, &pw[2 * i], &pw[2 * i + 1]);
+    }
+    pc->pw = pw;
+    pc->pwN = p->N;
     for (long k = j->k0; k < j->k1; ++k) {
         for (int m0 = 0; m0 < p->M; m0 += lanes) {
             int nl = p->M - m0 < lanes ? p->M - m0 : lanes;
@@ -394,6 +420,7 @@ static void *FN(worker)(void *arg) {
         }
     }
     free(pc);
+    free(pw);
     free(X1);
     free(x);
     return NULL;

@@ -18,8 +18,8 @@ def desc(prec, domain, dirn, place, M, Ns, K, istride=None, ostride=None):
 
 PRECS = ["s", "d"]
 
-# test/c2c.cpp:56-63 (M list trimmed of 1024 for the analytic test's host check time)
-C2C_KA_M = [1, 2, 3, 16, 17, 64, 256]
+# test/c2c.cpp:56-63
+C2C_KA_M = [1, 2, 3, 16, 17, 64, 256, 1024]
 C2C_KA_N = [2, 3, 5, 7, 11, 13, 4, 8, 16, 32, 128, 256, 512, 27, 63, 105, 363]
 C2C_KA_K = [1, 32]
 
@@ -61,6 +61,12 @@ def c2c_random():
                 for N in [64, 243]:
                     out.append(desc(p, "c", d, "o", M, [N], 7))
     return out
+
+
+def c1_full_k():
+    """The CPU configuration's shape, c2c M=1 x N=64 x K=16384 (full K), both precisions and
+    directions, against the FP64 oracle."""
+    return [desc(p, "c", d, "o", 1, [64], 16384) for p in PRECS for d in "fb"]
 
 
 def c2c_full_size():
@@ -147,8 +153,8 @@ def c3_descriptors():
 
 def all_descriptors():
     out = (c2c_known_answer() + c2c_nonpacked() + c2c_identity() + c2c_random() +
-           c2c_full_size() + r2c_1d_oop() + r2c_1d_inplace() + r2c_nd() + c2r_1d_oop() +
-           c2r_polluted() + c2r_nd() + c3_descriptors())
+           c1_full_k() + c2c_full_size() + r2c_1d_oop() + r2c_1d_inplace() + r2c_nd() +
+           c2r_1d_oop() + c2r_polluted() + c2r_nd() + c3_descriptors())
     for c in r2c_identity():
         out += identity_descriptors(c)
     seen, uniq = set(), []

@@ -67,6 +67,17 @@ def test_c2c_identity(desc):
     assert approx_equal(xr.imag, x.imag, eps).all()
 
 
+@pytest.mark.parametrize("desc", gpu_cases.c1_full_k())
+def test_c1_shape_full_k_vs_oracle(desc):
+    """M=1 x N=64 x K=16384 at full K, seeded uniform[-1,1) inputs, against the FP64 oracle."""
+    cfg = bb.parse_fft_descriptor(desc)
+    M, N, K = cfg.shape[:3]
+    x = oracle.uniform_complex(1, M * N * K)
+    y = execute(cfg, x)
+    ref = oracle.fft(1, cfg.shape, cfg.istride, cfg.ostride, 0, int(cfg.dir), x)
+    assert oracle.rel_l2(y, ref) <= oracle.tolerance(int(cfg.fp), N)
+
+
 @pytest.mark.parametrize("desc", gpu_cases.c2c_random())
 def test_c2c_random_vs_oracle(desc):
     """Seeded uniform[-1,1) inputs vs the FP64 DFT oracle: rel-L2 <= 8 eps log2 N."""

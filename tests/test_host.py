@@ -404,3 +404,19 @@ def test_plan_execute_checks_tensor_arguments():
         plan.execute(x, x)
     with pytest.raises(ValueError, match="GPU tensor"):
         plan.execute(x)
+
+
+def test_ragged_plan_checks_tensor_arguments():
+    """Ragged plans check tensor arguments against the jobs' extents like plain plans."""
+    import ctypes
+    import torch
+    from bbfft_amd import ragged
+    jobs = [ragged.Job(M=2, N=60, K=10, in_offset=0, out_offset=5000),
+            ragged.Job(M=1, N=12, K=100, in_offset=1200, out_offset=0)]
+    assert ragged.io_bytes(jobs, 4) == ((1200 + 1200) * 8, (5000 + 1200) * 8)
+    assert ragged.io_bytes(jobs, 8) == ((1200 + 1200) * 16, (5000 + 1200) * 16)
+    plan = bb.Plan(ctypes.c_void_p(), None, 0, io_bytes=ragged.io_bytes(jobs, 4))
+    x = torch.empty(6200, dtype=torch.complex64)
+    with pytest.raises(ValueError, match="GPU tensor"):
+        plan.execute(x, x)
+    plan._h = None
