@@ -29,16 +29,27 @@ namespace {
 std::size_t padded(std::size_t n, std::size_t pad) { return pad ? n + n / pad : n; }
 
 bool needs_lds(fft1d_configuration const &c) {
-    return c.factorization.size() > 1 || c.type != transform_type::c2c;
+    return c.factorization.size() > 1 || c.type != transform_type::c2c || c.direct_radix();
 }
 } // namespace
+
+int fft1d_configuration::direct_radix() const {
+    for (auto f : factorization) {
+        if (is_direct_radix(f)) {
+            return f;
+        }
+    }
+    return 0;
+}
 
 std::size_t fft1d_configuration::lds_bytes() const {
     if (!needs_lds(*this)) {
         return 0;
     }
     std::size_t np = padded(N_fft() - 1, lds_pad) + 1;
-    return Mb * Kb * np * 2 * static_cast<std::size_t>(fp);
+    // the direct stage keeps its table of roots of unity in LDS
+    std::size_t const table = static_cast<std::size_t>(direct_radix());
+    return (Mb * Kb * np + table) * 2 * static_cast<std::size_t>(fp);
 }
 
 std::string fft1d_configuration::identifier() const {
@@ -210,17 +221,50 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
             c.factorization.insert(c.factorization.begin(), 16);
         }
     }
+    // Radices above the register limit run as a direct DFT stage (one per kernel, odd radix:
+    // the symmetric-pair sums need no self-paired index), as the reference falls back to
+    // basic_esum for prime factors above its register budget.
+    int direct = 0;
     for (auto f : c.factorization) {
-        if (f > 64 && f != 1) {
-            throw bad_configuration("Radix " + std::to_string(f) +
-                                    " exceeds the in-register limit (64); N = " +
-                                    std::to_string(N) + " is not supported");
+        if (is_direct_radix(f)) {
+            if (f % 2 == 0) {
+                throw bad_configuration("Radix " + std::to_string(f) +
+                                        " exceeds the in-register limit (" +
+                                        std::to_string(fft1d_max_register_radix) +
+                                        ") and is even; split it into smaller radices");
+            }
+            if (direct) {
+                throw bad_configuration("N = " + std::to_string(N) +
+                                        " has two factors above the in-register limit (" +
+                                        std::to_string(fft1d_max_register_radix) +
+                                        "); at most one is supported");
+            }
+            direct = f;
         }
+    }
+    if (direct && mfma) {
+        throw bad_configuration("The MFMA stage does not combine with a direct DFT stage");
     }
     // The MFMA stage does not use the work-items-per-transform split; later stages set it.
     int const rmax = mfma ? *std::max_element(c.factorization.begin() + 1, c.factorization.end())
                           : *std::max_element(c.factorization.begin(), c.factorization.end());
-    c.Nt = tuning.Nt ? tuning.Nt : nfft / static_cast<std::size_t>(rmax);
+    if (direct) {
+        // Work-item t of a transform serves butterfly t % (N_fft / R) of the direct stage and
+        // output pairs t / (N_fft / R) + b Tk of it; Tk is chosen so that each work-item keeps
+        // about 8 (fp32) or 6 (fp64) output pairs in registers.
+        std::size_t const NRd = nfft / static_cast<std::size_t>(direct);
+        std::size_t const units = static_cast<std::size_t>(direct - 1) / 2 + 1;
+        std::size_t const per_item = cfg.fp == precision::f32 ? 8 : 6;
+        std::size_t const Tk = (units + per_item - 1) / per_item;
+        c.Nt = tuning.Nt ? tuning.Nt : NRd * Tk;
+        if (c.Nt % NRd != 0) {
+            throw bad_configuration("With a direct DFT stage of radix " + std::to_string(direct) +
+                                    " the work-items per transform must be a multiple of " +
+                                    std::to_string(NRd));
+        }
+    } else {
+        c.Nt = tuning.Nt ? tuning.Nt : nfft / static_cast<std::size_t>(rmax);
+    }
 
     std::size_t const wg_max = std::min<std::size_t>(1024, info.max_work_group_size);
     if (c.Nt == 0 || c.Nt > wg_max) {
@@ -291,9 +335,10 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     c.waves_per_eu = tuning.waves_per_eu;
     c.groups_per_cu = tuning.groups_per_cu;
     if (c.flags & fft1d_prefetch) {
-        if (cfg.type == transform_type::c2r || cfg.callbacks || (c.flags & fft1d_mfma)) {
-            throw bad_configuration(
-                "Tile prefetching requires a c2c or r2c kernel without callbacks or MFMA stage");
+        if (cfg.type == transform_type::c2r || cfg.callbacks || (c.flags & fft1d_mfma) ||
+            direct) {
+            throw bad_configuration("Tile prefetching requires a c2c or r2c kernel without "
+                                    "callbacks, MFMA or direct DFT stage");
         }
     }
     if ((c.flags & fft1d_adjoint) && cfg.type == transform_type::c2c) {

@@ -326,7 +326,8 @@ class kernel_gen {
         real_ = c.type != transform_type::c2c;
         half_ = real_ && c.N % 2 == 0;
         pairs_ = real_ && c.N % 2 == 1;
-        use_lds_ = L_ > 1 || real_;
+        direct_ = c.direct_radix();
+        use_lds_ = L_ > 1 || real_ || direct_;
         MG_ = (c.M + c.Mb - 1) / c.Mb;
         if (c.lds_pad) {
             pad_shift_ = 0;
@@ -348,7 +349,7 @@ class kernel_gen {
         bf16_ = (c.flags & fft1d_input_bf16) != 0;
         mfma_ = (c.flags & fft1d_mfma) != 0;
         prefetch_ = (c.flags & fft1d_prefetch) != 0 && local32_ && !mfma_ &&
-                    c.type != transform_type::c2r;
+                    c.type != transform_type::c2r && !direct_;
     }
 
     std::string generate() {
@@ -364,6 +365,10 @@ class kernel_gen {
     /// persistent ragged kernel. Work-items beyond Mb * Nt * Kb of a `group_size` work-group
     /// take part in the barriers only.
     std::string generate_body() {
+        if (direct_) {
+            throw bad_configuration("Tile bodies (ragged and fused kernels) support register "
+                                    "radices only");
+        }
         prefetch_ = false; // one tile per call: nothing to pipeline
         emit_tables();
         emit_body();
@@ -436,6 +441,7 @@ class kernel_gen {
 
     std::string tw_name(std::size_t s) const { return name_ + "_tw" + std::to_string(s); }
     std::string twr_name() const { return name_ + "_twr"; }
+    std::string dtab_name() const { return name_ + "_dtab"; }
 
     void emit_table(std::string const &name, std::vector<std::pair<long, long>> const &roots,
                     bool times_i) {
@@ -509,6 +515,14 @@ class kernel_gen {
             emit_table(tw_name(s), roots, false);
             p *= R;
         }
+        if (direct_) {
+            // (cos, sin)(2 pi m / R), m = 0..R-1, of the direct stage
+            std::vector<std::pair<long, long>> roots;
+            for (long m = 0; m < direct_; ++m) {
+                roots.push_back({m, direct_});
+            }
+            emit_table(dtab_name(), roots, false);
+        }
         if (half_) {
             // i * w_N^{dir q}, q = 0..H/2 (reference: sbfft_gen.cpp:192,238)
             std::vector<std::pair<long, long>> roots;
@@ -536,6 +550,13 @@ class kernel_gen {
                     "];");
         }
         w_.line("const unsigned lid = threadIdx.x;");
+        if (direct_) {
+            // roots of the direct stage in LDS (read with per-lane indices); the first tile's
+            // barrier after the direct stage's input phase orders these writes before any read
+            w_.line("__shared__ " + T2() + " dts[" + std::to_string(direct_) + "];");
+            w_.line("for (unsigned q = lid; q < " + u(direct_) + "; q += " + u(wg) + ") dts[q] = " +
+                    dtab_name() + "[q];");
+        }
         w_.line("const unsigned ml = lid % " + u(c_.Mb) + ";");
         w_.line("const unsigned t = (lid / " + u(c_.Mb) + ") % " + u(c_.Nt) + ";");
         w_.line("const unsigned kl = lid / " + u(c_.Mb * c_.Nt) + ";");
@@ -609,6 +630,8 @@ class kernel_gen {
         for (std::size_t s = 0; s < L_; ++s) {
             if (s == 0 && mfma_) {
                 emit_mfma_stage0();
+            } else if (is_direct_radix(c_.factorization[s])) {
+                emit_direct_stage(s);
             } else {
                 emit_stage(s);
             }
@@ -1077,6 +1100,201 @@ class kernel_gen {
         }
     }
 
+    /**
+     * Stage s with an odd radix R above the register limit as a direct DFT (the reference's
+     * basic_esum fallback for large prime factors, src/base/mixed_radix_fft.cpp:46-53, in the
+     * symmetric-pair form of its pair optimisation, mixed_radix_fft.cpp:56-118):
+     *   y_k = x_0 + sum_i a_i cos(2 pi i k / R) + dir i sum_i b_i sin(2 pi i k / R),
+     *   y_{R-k} = the same with the sine sum negated, a_i = x_i + x_{R-i}, b_i = x_i - x_{R-i}.
+     * Phase 1: units (j, i), i = 0..h = (R-1)/2, read x_i and x_{R-i} of butterfly j (global
+     * memory or LDS, twiddled by w_{pR}^{i (j mod p)} for s > 0) and leave x_0, a_i, b_i in the
+     * butterfly's LDS slots j + i NR and j + (R - i) NR. Phase 2: work-item t serves butterfly
+     * j = t mod NR and the output pairs (k, R - k), k = t / NR + b Tk, accumulating the sums
+     * over i in registers (two partial sums in fp32) with the roots read from an LDS table at
+     * (i k) mod R; after a barrier the outputs go to LDS (Stockham order) or global memory.
+     */
+    void emit_direct_stage(std::size_t s) {
+        std::size_t const R = c_.factorization[s];
+        std::size_t p = 1;
+        for (std::size_t i = 0; i < s; ++i) {
+            p *= c_.factorization[i];
+        }
+        std::size_t const NR = nfft_ / R;
+        std::size_t const h = (R - 1) / 2;
+        std::size_t const U = h + 1;      // output units (k, R - k) per butterfly, k = 0..h
+        std::size_t const Tk = c_.Nt / NR; // work-items per butterfly
+        std::size_t const Bu = (U + Tk - 1) / Tk;
+        std::size_t const parts = c_.fp == precision::f32 ? 2 : 1;
+        bool const src_global = s == 0 && c_.type != transform_type::c2r;
+        bool const dst_global = s + 1 == L_ && c_.type != transform_type::r2c;
+        std::string const S = std::to_string(s);
+        w_.line("// stage " + S + ": radix " + std::to_string(R) + " as a direct DFT, p = " +
+                std::to_string(p));
+        // phase 1
+        std::size_t const P1 = NR * U;
+        std::size_t const B1 = (P1 + c_.Nt - 1) / c_.Nt;
+        for (std::size_t b = 0; b < B1; ++b) {
+            std::string const x = S + "_" + std::to_string(b);
+            w_.open("");
+            w_.line("const unsigned du = t + " + u(b * c_.Nt) + ";");
+            std::string cond = (b + 1) * c_.Nt > P1 ? "du < " + u(P1) : "";
+            if (cond.empty()) {
+                w_.open("");
+            } else {
+                w_.open("if (" + cond + ")");
+            }
+            w_.line("const unsigned dj = " + (NR == 1 ? std::string("0u") : "du % " + u(NR)) +
+                    ";");
+            w_.line("const unsigned di = " + (NR == 1 ? std::string("du") : "du / " + u(NR)) +
+                    ";");
+            // x_{R-i} of unit i = 0 does not exist: its index falls back to x_0's
+            w_.line("const unsigned dn1 = dj + di * " + u(NR) + ";");
+            w_.line("const unsigned dn2 = di == 0u ? dj : dj + (" + u(R) + " - di) * " + u(NR) +
+                    ";");
+            w_.line(T2() + " x1 = " + f_.zero2() + ", x2 = " + f_.zero2() + ";");
+            if (src_global) {
+                w_.open("if (active)");
+                emit_global_input("x1", "dn1");
+                emit_global_input("x2", "dn2");
+                w_.close();
+            } else {
+                w_.line("x1 = " + lds("dn1") + ";");
+                w_.line("x2 = " + lds("dn2") + ";");
+            }
+            if (s > 0) {
+                std::string jp = "(dj % " + u(p) + ") * " + u(R);
+                w_.line("x1 = c_mul(x1, " + tw_name(s) + "[" + jp + " + di]);");
+                w_.line("x2 = c_mul(x2, " + tw_name(s) + "[" + jp + " + (di == 0u ? 0u : " + u(R) +
+                        " - di)]);");
+            }
+            w_.open("if (di == 0u)");
+            w_.line(lds("dn1") + " = x1;");
+            w_.close();
+            w_.open("else");
+            w_.line(lds("dn1") + " = c_add(x1, x2);");
+            w_.line(lds("dn2") + " = c_sub(x1, x2);");
+            w_.close();
+            w_.close();
+            w_.close();
+        }
+        w_.line("__syncthreads();");
+        // phase 2
+        w_.open("");
+        w_.line("const unsigned dj = " + (NR == 1 ? std::string("0u") : "t % " + u(NR)) + ";");
+        w_.line("const unsigned dk = " + (NR == 1 ? std::string("t") : "t / " + u(NR)) + ";");
+        w_.line("const " + T2() + " x0 = " + lds("dj") + ";");
+        std::vector<std::string> kk(Bu), valid(Bu);
+        for (std::size_t b = 0; b < Bu; ++b) {
+            std::string const x = std::to_string(b);
+            kk[b] = "dkk" + x;
+            w_.line("const unsigned dk" + x + " = dk + " + u(b * Tk) + ";");
+            if ((b + 1) * Tk > U) {
+                valid[b] = "dk" + x + " < " + u(U);
+                // invalid units run with k = 0 (in-bounds table reads), their sums are unused
+                w_.line("const unsigned " + kk[b] + " = " + valid[b] + " ? dk" + x + " : 0u;");
+            } else {
+                w_.line("const unsigned " + kk[b] + " = dk" + x + ";");
+            }
+            w_.line("unsigned dm" + x + " = 0u;");
+            for (std::size_t q = 0; q < parts; ++q) {
+                std::string const y = x + "_" + std::to_string(q);
+                w_.line(T2() + " dC" + y + " = " + f_.zero2() + ", dS" + y + " = " + f_.zero2() +
+                        ";");
+            }
+        }
+        auto body = [&](std::string const &i, std::size_t q) {
+            std::string const Q = std::to_string(q);
+            w_.line("const " + T2() + " da" + Q + " = " + lds("dj + (" + i + ") * " + u(NR)) +
+                    ";");
+            w_.line("const " + T2() + " db" + Q + " = " +
+                    lds("dj + (" + u(R) + " - (" + i + ")) * " + u(NR)) + ";");
+            for (std::size_t b = 0; b < Bu; ++b) {
+                std::string const x = std::to_string(b);
+                std::string const y = x + "_" + Q;
+                w_.line("dm" + x + " += " + kk[b] + ";");
+                w_.line("dm" + x + " = dm" + x + " >= " + u(R) + " ? dm" + x + " - " + u(R) +
+                        " : dm" + x + ";");
+                w_.line("const " + T2() + " dw" + y + " = dts[dm" + x + "];");
+                w_.line("dC" + y + " = mk(dC" + y + ".x + da" + Q + ".x * dw" + y + ".x, dC" + y +
+                        ".y + da" + Q + ".y * dw" + y + ".x);");
+                w_.line("dS" + y + " = mk(dS" + y + ".x + db" + Q + ".x * dw" + y + ".y, dS" + y +
+                        ".y + db" + Q + ".y * dw" + y + ".y);");
+            }
+        };
+        std::size_t const full = h / parts * parts; // i = 1..full in steps of `parts`
+        if (full > 0) {
+            w_.open("for (unsigned di = 1u; di <= " + u(full) + "; di += " + u(parts) + ")");
+            for (std::size_t q = 0; q < parts; ++q) {
+                w_.open("");
+                body(q ? "di + " + u(q) : "di", q);
+                w_.close();
+            }
+            w_.close();
+        }
+        for (std::size_t i = full + 1; i <= h; ++i) {
+            w_.open("");
+            body(u(i), 0);
+            w_.close();
+        }
+        // combine: y_k = x_0 + C + dir i S, y_{R-k} = x_0 + C - dir i S
+        for (std::size_t b = 0; b < Bu; ++b) {
+            std::string const x = std::to_string(b);
+            std::string C = "dC" + x + "_0", Sv = "dS" + x + "_0";
+            if (parts == 2) {
+                w_.line("const " + T2() + " dCt" + x + " = c_add(dC" + x + "_0, dC" + x + "_1);");
+                w_.line("const " + T2() + " dSt" + x + " = c_add(dS" + x + "_0, dS" + x + "_1);");
+                C = "dCt" + x;
+                Sv = "dSt" + x;
+            }
+            w_.line("const " + T2() + " dc" + x + " = c_add(x0, " + C + ");");
+            std::string const c = "dc" + x;
+            std::string plus = "mk(" + c + ".x - " + Sv + ".y, " + c + ".y + " + Sv + ".x)";
+            std::string minus = "mk(" + c + ".x + " + Sv + ".y, " + c + ".y - " + Sv + ".x)";
+            // forward (dir = -1): y_k = C - i S
+            w_.line("const " + T2() + " dyk" + x + " = " + (c_.direction < 0 ? minus : plus) +
+                    ";");
+            w_.line("const " + T2() + " dyr" + x + " = " + (c_.direction < 0 ? plus : minus) +
+                    ";");
+        }
+        // the sums read every slot of the butterfly: all reads end before the outputs land
+        if (!dst_global || alias_out_) {
+            w_.line("__syncthreads();");
+        }
+        auto out_index = [&](std::string const &k) {
+            return p == 1 ? "dj * " + u(R) + " + " + k
+                          : "(dj / " + u(p) + ") * " + u(p * R) + " + dj % " + u(p) + " + (" + k +
+                                ") * " + u(p);
+        };
+        for (std::size_t b = 0; b < Bu; ++b) {
+            std::string const x = std::to_string(b);
+            std::string const k = "dk" + x;
+            std::string cond = valid[b];
+            if (dst_global) {
+                cond = cond.empty() ? "active" : "active && " + cond;
+            }
+            if (cond.empty()) {
+                w_.open("");
+            } else {
+                w_.open("if (" + cond + ")");
+            }
+            if (dst_global) {
+                emit_global_output("dyk" + x, out_index(k));
+                w_.open("if (" + k + " != 0u)");
+                emit_global_output("dyr" + x, out_index(u(R) + " - " + k));
+                w_.close();
+            } else {
+                w_.line(lds(out_index(k)) + " = dyk" + x + ";");
+                w_.line("if (" + k + " != 0u) " + lds(out_index(u(R) + " - " + k)) + " = dyr" + x +
+                        ";");
+            }
+            w_.close();
+        }
+        w_.close();
+        if (!dst_global) {
+            w_.line("__syncthreads();");
+        }
+    }
+
     /// c2r: Hermitian input -> complex FFT input z in LDS (reference: sbfft_gen.cpp:221-247
     /// for even N, 318-351 for odd N; the imaginary part of the DC bin is ignored).
     void emit_c2r_pre() {
@@ -1230,6 +1448,7 @@ class kernel_gen {
     unsigned pad_shift_ = 0;
     bool real_ = false, half_ = false, pairs_ = false, use_lds_ = false;
     bool half_in_ = false, bf16_ = false, mfma_ = false;
+    int direct_ = 0; // radix of the direct DFT stage (0: none)
     bool local32_ = false;
     bool prefetch_ = false;
 };

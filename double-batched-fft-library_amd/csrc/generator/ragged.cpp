@@ -150,6 +150,12 @@ ragged_kernel_configuration configure_ragged(ragged_configuration const &cfg,
         if (c.work_group_size() > G) {
             throw bad_configuration("Ragged variant exceeds the common work-group size");
         }
+        if (c.direct_radix()) {
+            throw bad_configuration("Ragged job length " + std::to_string(N) +
+                                    " needs a direct DFT stage (a prime factor above " +
+                                    std::to_string(fft1d_max_register_radix) +
+                                    "); ragged batches support register radices only");
+        }
         rk.variants.push_back(c);
     }
     rk.group_size = G;

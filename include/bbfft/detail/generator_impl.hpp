@@ -61,8 +61,10 @@ struct BBFFT_EXPORT fft1d_configuration {
     /// Number of K-columns (or K-column pairs) for a given K.
     std::size_t K_columns(std::size_t K) const;
     std::size_t work_group_size() const { return Mb * Nt * Kb; }
-    /// Bytes of LDS used by one work-group.
+    /// Bytes of LDS used by one work-group (stage exchange plus direct-stage root tables).
     std::size_t lds_bytes() const;
+    /// Radix of the direct DFT stage (a radix above fft1d_max_register_radix), 0 if none.
+    int direct_radix() const;
 };
 
 /// Bits of fft1d_configuration::flags.
@@ -93,6 +95,12 @@ enum fft1d_flag : unsigned {
     /// not with fft1d_mfma.
     fft1d_prefetch = 64u,
 };
+
+/// Largest radix a stage computes in registers; a larger (odd) radix runs as a direct DFT stage
+/// whose butterfly is spread over several work-items (inputs in LDS, O(R^2) symmetric-pair
+/// sums, the role of the reference's basic_esum fallback, src/base/mixed_radix_fft.cpp:46-53).
+constexpr int fft1d_max_register_radix = 64;
+inline bool is_direct_radix(int r) { return r > fft1d_max_register_radix; }
 
 /// Number of half-precision terms the DFT matrix is split into on the MFMA stage.
 inline unsigned mfma_split_terms(unsigned flags) { return (flags & fft1d_input_bf16) ? 3u : 2u; }

@@ -69,6 +69,27 @@ def c1_full_k():
     return [desc(p, "c", d, "o", 1, [64], 16384) for p in PRECS for d in "fb"]
 
 
+# lengths with a prime factor above the in-register radix limit (direct DFT stage)
+DIRECT_N = [67, 134, 254, 262, 509]
+
+
+def direct_stage():
+    """c2c both directions, r2c and c2r, both precisions, at lengths needing a direct stage."""
+    out = []
+    for p in PRECS:
+        for N in DIRECT_N:
+            for M in [1, 8]:
+                out += [desc(p, "c", d, "o", M, [N], 33) for d in "fb"]
+            out += [desc(p, "r", "f", "o", 3, [N], 33), desc(p, "r", "b", "o", 3, [N], 33)]
+        out += [desc(p, "c", "f", "o", 2, [134, 6], 3), desc(p, "r", "f", "o", 2, [254, 5], 3),
+                desc(p, "r", "b", "o", 2, [262, 5], 3)]
+    return out
+
+
+def direct_full_size():
+    return [desc(p, "c", d, "o", 8, [509], 32768) for p in PRECS for d in "fb"]
+
+
 def c2c_full_size():
     return [desc(p, "c", d, "o", 8, [N], 32768) for p in PRECS for d in "fb" for N in BENCH_N]
 
@@ -153,7 +174,7 @@ def c3_descriptors():
 
 def all_descriptors():
     out = (c2c_known_answer() + c2c_nonpacked() + c2c_identity() + c2c_random() +
-           c1_full_k() + c2c_full_size() + r2c_1d_oop() + r2c_1d_inplace() + r2c_nd() +
+           c1_full_k() + direct_stage() + direct_full_size() + c2c_full_size() + r2c_1d_oop() + r2c_1d_inplace() + r2c_nd() +
            c2r_1d_oop() + c2r_polluted() + c2r_nd() + c3_descriptors())
     for c in r2c_identity():
         out += identity_descriptors(c)
