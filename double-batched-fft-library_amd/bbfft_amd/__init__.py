@@ -400,9 +400,11 @@ class Plan:
             else io_bytes
         self.stream = stream
 
-    def execute(self, inp, out=None, signal=None, wait: Sequence = ()):
+    def execute(self, inp, out=None, signal=None, wait: Sequence = (), static_scratch=False):
         """Out-of-place if out is given, else in-place. Events are hipEvent_t handles (int).
-        Tensors are checked for device and size (see _check_buffer); integer pointers are not."""
+        Tensors are checked for device and size (see _check_buffer); integer pointers are not.
+        static_scratch=True runs bbfft_plan_execute_static_scratch (allocates nothing; plans
+        needing a scratch buffer use the library's 64 MiB per-GPU array)."""
         if self._io is not None:
             ib, ob = self._io
             if out is None:
@@ -414,7 +416,9 @@ class Plan:
         dst = src if out is None else _ptr(out)
         nw = len(wait)
         warr = (ctypes.c_void_p * max(1, nw))(*[int(w) for w in wait]) if nw else None
-        _check(_capi.load().bbfft_plan_execute(self._h, src, dst, signal, nw, warr))
+        L = _capi.load()
+        fn = L.bbfft_plan_execute_static_scratch if static_scratch else L.bbfft_plan_execute
+        _check(fn(self._h, src, dst, signal, nw, warr))
 
     def launches(self):
         L = _capi.load()

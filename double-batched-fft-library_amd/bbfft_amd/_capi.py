@@ -28,7 +28,7 @@ EXPORTED = [
     "bbfft_ragged_plan_create", "bbfft_aot_generate_ragged", "bbfft_plan_create_half_input",
     "bbfft_fused_configuration", "bbfft_generate_fused_source",
     "bbfft_half_input_kernel_configuration", "bbfft_cache_load_aot_memory",
-    "bbfft_aot_archive_kernels",
+    "bbfft_aot_archive_kernels", "bbfft_plan_execute_static_scratch",
 ]
 
 SUCCESS = 0
@@ -170,6 +170,8 @@ def load():
                                                       P(c_void_p)]),
         "bbfft_plan_execute": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint32,
                                        P(c_void_p)]),
+        "bbfft_plan_execute_static_scratch": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p,
+                                                      ctypes.c_uint32, P(c_void_p)]),
         "bbfft_plan_num_launches": (c_int, [c_void_p, P(c_uint)]),
         "bbfft_plan_launch_info": (c_int, [c_void_p, c_uint, c_char_p, c_size, P(c_uint),
                                            P(c_uint), P(c_size)]),

@@ -5,6 +5,7 @@
 #include "../generator/roots.hpp"
 #include "../hip/aot_archive.hpp"
 #include "../hip/hip_plan_impl.hpp"
+#include "../hip/static_scratch.hpp"
 #include "../plan/decompose.hpp"
 #include "bbfft/aot_cache.hpp"
 #include "bbfft/bad_configuration.hpp"
@@ -487,6 +488,35 @@ int bbfft_plan_execute(bbfft_plan_t plan, const void *in, void *out, bbfft_event
         }
         plan->impl->execute(in, out, static_cast<hipEvent_t>(signal), num_wait,
                             num_wait ? w.data() : nullptr);
+    });
+}
+
+int bbfft_plan_execute_static_scratch(bbfft_plan_t plan, const void *in, void *out,
+                                      bbfft_event_t signal, uint32_t num_wait,
+                                      const bbfft_event_t *wait) {
+    return guarded([&] {
+        require(plan != nullptr, "null plan");
+        require(num_wait == 0 || wait != nullptr, "wait events missing");
+        std::size_t const need = plan->impl->scratch_bytes();
+        if (need > hip_detail::static_scratch_bytes) {
+            throw bad_configuration(
+                "The plan needs " + std::to_string(need) +
+                " bytes of scratch memory, more than the static scratch array holds (" +
+                std::to_string(hip_detail::static_scratch_bytes) +
+                " bytes); use bbfft_plan_execute");
+        }
+        void *scratch = nullptr;
+        if (need) {
+            scratch = hip_detail::static_scratch(plan->impl->device());
+        }
+        hipStream_t const stream = plan->impl->stream();
+        for (uint32_t i = 0; i < num_wait; ++i) {
+            BBFFT_HIP_CHECK(hipStreamWaitEvent(stream, static_cast<hipEvent_t>(wait[i]), 0));
+        }
+        plan->impl->execute_with_scratch(in, out, scratch);
+        if (signal) {
+            BBFFT_HIP_CHECK(hipEventRecord(static_cast<hipEvent_t>(signal), stream));
+        }
     });
 }
 

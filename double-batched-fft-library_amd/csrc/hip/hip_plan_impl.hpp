@@ -29,6 +29,13 @@ struct launch_info {
 class plan_base : public detail::plan_unmanaged_event_impl<hipEvent_t> {
   public:
     virtual std::vector<launch_info> launches() const = 0;
+    /// Same as execute (without events) with `scratch` (at least scratch_bytes() bytes of
+    /// device memory) in place of the plan's own scratch buffer; allocates nothing.
+    virtual void execute_with_scratch(void const *in, void *out, void *scratch) = 0;
+    /// Device memory the plan needs between its passes (0: none).
+    virtual std::size_t scratch_bytes() const { return 0; }
+    virtual hipStream_t stream() const = 0;
+    virtual int device() const = 0;
 };
 
 /// A single 1-D double-batched kernel.
@@ -40,6 +47,9 @@ class fft1d_plan final : public plan_base {
                  hipEvent_t *wait) override;
     /// Launch on the plan's stream without event handling.
     void launch(void const *in, void *out);
+    void execute_with_scratch(void const *in, void *out, void *) override { launch(in, out); }
+    hipStream_t stream() const override { return stream_; }
+    int device() const override { return device_; }
     std::vector<launch_info> launches() const override;
     fft1d_configuration const &kernel_config() const { return kc_; }
 
@@ -82,14 +92,22 @@ class nd_plan final : public plan_base {
     nd_plan &operator=(nd_plan const &) = delete;
     void execute(void const *in, void *out, hipEvent_t signal, std::uint32_t nwait,
                  hipEvent_t *wait) override;
+    void execute_with_scratch(void const *in, void *out, void *scratch) override;
+    std::size_t scratch_bytes() const override { return tmp_bytes_; }
+    hipStream_t stream() const override { return stream_; }
+    int device() const override { return device_; }
     std::vector<launch_info> launches() const override;
 
   private:
+    void run(void const *in, void *out, void *tmp);
+
     hipStream_t stream_;
+    int device_;
     std::vector<std::unique_ptr<fft1d_plan>> passes_; ///< null where a fused kernel runs
     std::unique_ptr<fused2_kernel> fused_;
     std::size_t fused_first_ = static_cast<std::size_t>(-1);
     void *tmp_ = nullptr;
+    std::size_t tmp_bytes_ = 0;
 };
 
 /// Ragged batch: one persistent kernel over a device job table, optionally replayed from a
@@ -103,6 +121,10 @@ class ragged_plan final : public plan_base {
     ragged_plan &operator=(ragged_plan const &) = delete;
     void execute(void const *in, void *out, hipEvent_t signal, std::uint32_t nwait,
                  hipEvent_t *wait) override;
+    /// Direct launch (no graph capture, which may allocate).
+    void execute_with_scratch(void const *in, void *out, void *) override;
+    hipStream_t stream() const override { return stream_; }
+    int device() const override { return device_; }
     std::vector<launch_info> launches() const override;
     ragged_kernel_configuration const &kernel_config() const { return rk_; }
 

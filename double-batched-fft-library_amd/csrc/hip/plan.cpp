@@ -169,7 +169,7 @@ launch_info fused2_kernel::info() const {
 
 nd_plan::nd_plan(pass_layout const &layout, hipStream_t stream, int device, jit_cache *cache,
                  unsigned real_pass_variant)
-    : stream_(stream) {
+    : stream_(stream), device_(device) {
     fused_pair fp;
     if (find_fused_pair(layout, hip::get_device_info(device), real_pass_variant, fp)) {
         fused_first_ = fp.first;
@@ -187,6 +187,7 @@ nd_plan::nd_plan(pass_layout const &layout, hipStream_t stream, int device, jit_
     }
     if (layout.tmp_bytes) {
         BBFFT_HIP_CHECK(hipMalloc(&tmp_, layout.tmp_bytes));
+        tmp_bytes_ = layout.tmp_bytes;
     }
 }
 
@@ -201,10 +202,26 @@ void nd_plan::execute(void const *in, void *out, hipEvent_t signal, std::uint32_
     for (std::uint32_t i = 0; i < nwait; ++i) {
         BBFFT_HIP_CHECK(hipStreamWaitEvent(stream_, wait[i], 0));
     }
+    run(in, out, tmp_);
+    if (signal) {
+        BBFFT_HIP_CHECK(hipEventRecord(signal, stream_));
+    }
+}
+
+void nd_plan::execute_with_scratch(void const *in, void *out, void *scratch) {
+    if (tmp_bytes_ && !scratch) {
+        throw bad_configuration("The plan needs a scratch buffer");
+    }
+    run(in, out, tmp_bytes_ ? scratch : nullptr);
+}
+
+void nd_plan::run(void const *in, void *out, void *tmp) {
     // All steps run on one stream, so stream order replaces the reference's internal events.
     // Step 0 reads `in`, the last step writes `out`, the others work in the scratch buffer (or
     // in `out` when the input is not larger than the output).
-    void *tmp = tmp_ ? tmp_ : out;
+    if (!tmp) {
+        tmp = out;
+    }
     std::size_t const D = passes_.size();
     std::size_t const steps = fused_ ? D - 1 : D;
     std::size_t step = 0;
@@ -217,9 +234,6 @@ void nd_plan::execute(void const *in, void *out, hipEvent_t signal, std::uint32_
         } else {
             passes_[i]->launch(src, dst);
         }
-    }
-    if (signal) {
-        BBFFT_HIP_CHECK(hipEventRecord(signal, stream_));
     }
 }
 

@@ -159,6 +159,15 @@ void ragged_plan::execute(void const *in, void *out, hipEvent_t signal, std::uin
     }
 }
 
+void ragged_plan::execute_with_scratch(void const *in, void *out, void *) {
+    std::size_t const align = 2 * static_cast<std::size_t>(rk_.fp);
+    if (reinterpret_cast<std::uintptr_t>(in) % align ||
+        reinterpret_cast<std::uintptr_t>(out) % align) {
+        throw bad_configuration("Input/output pointers are not aligned to the element size");
+    }
+    launch(in, out, stream_);
+}
+
 std::vector<launch_info> ragged_plan::launches() const {
     return {launch_info{name_, grid_, block_, rk_.lds_bytes()}};
 }

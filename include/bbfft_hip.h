@@ -182,6 +182,20 @@ BBFFT_EXPORT_C int bbfft_plan_create_tuned(const bbfft_configuration *cfg,
 BBFFT_EXPORT_C int bbfft_plan_execute(bbfft_plan_t plan, const void *in, void *out,
                                       bbfft_event_t signal, uint32_t num_wait,
                                       const bbfft_event_t *wait);
+/* Allocation-free variant of bbfft_plan_execute. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). It calls no hipMalloc/hipFree and no allocating
+ * library routine. Plans whose passes need a scratch buffer between them (2-D/3-D plans whose
+ * input is larger than their output, i.e. out-of-place c2r) use a file-scope device array of
+ * 64 MiB per GPU in place of the plan's own buffer; a plan needing more is rejected with
+ * BBFFT_ERROR_BAD_CONFIGURATION before anything is launched. Ragged plans launch directly
+ * (no graph capture). The array needs no clearing (every pass writes what the next reads),
+ * keeps the last call's contents, and is shared by every call on that GPU: calls that overlap
+ * on one GPU (e.g. on two streams) race on it. */
+BBFFT_EXPORT_C int bbfft_plan_execute_static_scratch(bbfft_plan_t plan, const void *in,
+                                                     void *out, bbfft_event_t signal,
+                                                     uint32_t num_wait,
+                                                     const bbfft_event_t *wait);
 /* Introspection of the kernels a plan launches. */
 BBFFT_EXPORT_C int bbfft_plan_num_launches(bbfft_plan_t plan, unsigned *num);
 BBFFT_EXPORT_C int bbfft_plan_launch_info(bbfft_plan_t plan, unsigned index, char *name,

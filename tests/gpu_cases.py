@@ -172,10 +172,18 @@ def c3_descriptors():
     return [desc("s", "r", d, "o", 1, [128, 128, 128], K) for d in "fb" for K in [32, 2]]
 
 
+def static_scratch():
+    """Plans run through the allocation-free execute entry point (pairs: larger, smaller)."""
+    return ["srbo128x96*40", "srbo24x20x12*6", "drbo64x48x10*5", "drbo16x12*9",
+            "srbo128x128x128*2", "srbo128x128*8", "scfo8.256*4096", "scbo1.60*77",
+            "srfo64x32*20", "srbi32x16x8*3", "srbo512x512*64", "srbo32x16*10"]
+
+
 def all_descriptors():
     out = (c2c_known_answer() + c2c_nonpacked() + c2c_identity() + c2c_random() +
-           c1_full_k() + direct_stage() + direct_full_size() + c2c_full_size() + r2c_1d_oop() + r2c_1d_inplace() + r2c_nd() +
-           c2r_1d_oop() + c2r_polluted() + c2r_nd() + c3_descriptors())
+           c1_full_k() + direct_stage() + direct_full_size() + c2c_full_size() +
+           r2c_1d_oop() + r2c_1d_inplace() + r2c_nd() +
+           c2r_1d_oop() + c2r_polluted() + c2r_nd() + c3_descriptors() + static_scratch())
     for c in r2c_identity():
         out += identity_descriptors(c)
     seen, uniq = set(), []
