@@ -318,9 +318,9 @@ class kernel_gen {
     /// work-group's LDS scratch (fused multi-dimensional kernels), so reads of the input and
     /// writes of the output are separated from scratch writes / reads by barriers.
     kernel_gen(fft1d_configuration const &c, std::string name, std::size_t group_size = 0,
-               bool alias_in = false, bool alias_out = false)
+               bool alias_in = false, bool alias_out = false, bool lds_tables = false)
         : c_(c), f_{c.fp}, name_(std::move(name)), group_size_(group_size), alias_in_(alias_in),
-          alias_out_(alias_out) {
+          alias_out_(alias_out), lds_tables_(lds_tables) {
         nfft_ = c.N_fft();
         L_ = c.factorization.size();
         real_ = c.type != transform_type::c2c;
@@ -439,8 +439,13 @@ class kernel_gen {
         return "tout[" + off + "] = " + val + ";";
     }
 
-    std::string tw_name(std::size_t s) const { return name_ + "_tw" + std::to_string(s); }
-    std::string twr_name() const { return name_ + "_twr"; }
+    /// Work-group barrier (BBFFT_SYNC: __syncthreads() unless the kernel's source defines it).
+    static std::string sync() { return "BBFFT_SYNC();"; }
+    /// Twiddle tables: __constant__ arrays, or their LDS copies (lds_tables_).
+    std::string tw_name(std::size_t s) const {
+        return name_ + "_tw" + std::to_string(s) + (lds_tables_ ? "_l" : "");
+    }
+    std::string twr_name() const { return name_ + "_twr" + (lds_tables_ ? "_l" : ""); }
     std::string dtab_name() const { return name_ + "_dtab"; }
 
     void emit_table(std::string const &name, std::vector<std::pair<long, long>> const &roots,
@@ -512,7 +517,7 @@ class kernel_gen {
                                      long(p * R)});
                 }
             }
-            emit_table(tw_name(s), roots, false);
+            emit_table(name_ + "_tw" + std::to_string(s), roots, false);
             p *= R;
         }
         if (direct_) {
@@ -529,8 +534,37 @@ class kernel_gen {
             for (std::size_t q = 0; q <= nfft_ / 2; ++q) {
                 roots.push_back({static_cast<long>(c_.direction) * long(q), long(c_.N)});
             }
-            emit_table(twr_name(), roots, true);
+            emit_table(name_ + "_twr", roots, true);
         }
+        if (lds_tables_) {
+            emit_lds_tables();
+        }
+    }
+
+    /// LDS copies of the twiddle tables and `NAME_tables(lid, n)`, which fills them with the
+    /// n work-items lid = 0..n-1 (the caller orders it before the first use with a barrier).
+    void emit_lds_tables() {
+        std::vector<std::pair<std::string, std::size_t>> tabs;
+        std::size_t p = c_.factorization[0];
+        for (std::size_t s = 1; s < L_; ++s) {
+            tabs.push_back({name_ + "_tw" + std::to_string(s), p * c_.factorization[s]});
+            p *= c_.factorization[s];
+        }
+        if (half_) {
+            tabs.push_back({name_ + "_twr", nfft_ / 2 + 1});
+        }
+        for (auto const &[t, n] : tabs) {
+            w_.line("__shared__ " + T2() + " " + t + "_l[" + std::to_string(n) + "];");
+        }
+        w_.line("static __device__ __forceinline__ void " + name_ +
+                "_tables(unsigned lid, unsigned n) {");
+        w_.open("");
+        for (auto const &[t, n] : tabs) {
+            w_.line("for (unsigned q = lid; q < " + u(n) + "; q += n) " + t + "_l[q] = " + t +
+                    "[q];");
+        }
+        w_.close();
+        w_.line("}");
     }
 
     // ---- kernel ----------------------------------------------------------------------------
@@ -587,7 +621,7 @@ class kernel_gen {
         w_.open("for (unsigned long tile = blockIdx.x; tile < ntiles; tile += gridDim.x)");
         emit_tile();
         if (use_lds_) {
-            w_.line("__syncthreads();");
+            w_.line(sync());
         }
         w_.close();
         w_.close();
@@ -625,7 +659,7 @@ class kernel_gen {
         emit_tile_origin();
         if (c_.type == transform_type::c2r) {
             emit_c2r_pre();
-            w_.line("__syncthreads();");
+            w_.line(sync());
         }
         for (std::size_t s = 0; s < L_; ++s) {
             if (s == 0 && mfma_) {
@@ -849,7 +883,7 @@ class kernel_gen {
             w_.close();
         }
         w_.close();
-        w_.line("__syncthreads();");
+        w_.line(sync());
     }
 
     /// Offset (elements) of FFT index n in the input column along mode 1.
@@ -1019,7 +1053,7 @@ class kernel_gen {
             w_.close();
         }
         if (mid_barrier) {
-            w_.line("__syncthreads();");
+            w_.line(sync());
         }
         for (std::size_t b = 0; b < B; ++b) {
             std::string cond = guard[b];
@@ -1096,7 +1130,7 @@ class kernel_gen {
             w_.close();
         }
         if (!dst_global) {
-            w_.line("__syncthreads();");
+            w_.line(sync());
         }
     }
 
@@ -1177,7 +1211,7 @@ class kernel_gen {
             w_.close();
             w_.close();
         }
-        w_.line("__syncthreads();");
+        w_.line(sync());
         // phase 2
         w_.open("");
         w_.line("const unsigned dj = " + (NR == 1 ? std::string("0u") : "t % " + u(NR)) + ";");
@@ -1258,7 +1292,7 @@ class kernel_gen {
         }
         // the sums read every slot of the butterfly: all reads end before the outputs land
         if (!dst_global || alias_out_) {
-            w_.line("__syncthreads();");
+            w_.line(sync());
         }
         auto out_index = [&](std::string const &k) {
             return p == 1 ? "dj * " + u(R) + " + " + k
@@ -1291,7 +1325,7 @@ class kernel_gen {
         }
         w_.close();
         if (!dst_global) {
-            w_.line("__syncthreads();");
+            w_.line(sync());
         }
     }
 
@@ -1343,7 +1377,7 @@ class kernel_gen {
             w_.close();
         }
         if (alias_in_) {
-            w_.line("__syncthreads();");
+            w_.line(sync());
         }
         // phase 2: store them in LDS
         for (std::size_t b = 0; b < B; ++b) {
@@ -1417,7 +1451,7 @@ class kernel_gen {
             w_.close();
         }
         if (alias_out_) {
-            w_.line("__syncthreads();");
+            w_.line(sync());
         }
         // phase 2: store the half spectrum (q = 0 also writes the Nyquist / last bin H)
         for (std::size_t b = 0; b < B; ++b) {
@@ -1448,6 +1482,7 @@ class kernel_gen {
     unsigned pad_shift_ = 0;
     bool real_ = false, half_ = false, pairs_ = false, use_lds_ = false;
     bool half_in_ = false, bf16_ = false, mfma_ = false;
+    bool lds_tables_ = false; // twiddles read from LDS copies (fused kernels)
     int direct_ = 0; // radix of the direct DFT stage (0: none)
     bool local32_ = false;
     bool prefetch_ = false;
@@ -1459,6 +1494,7 @@ class kernel_gen {
 std::string hip_prelude() {
     std::ostringstream os;
     os << "#ifndef BBFFT_HIP_PRELUDE\n#define BBFFT_HIP_PRELUDE\n";
+    os << "#ifndef BBFFT_SYNC\n#define BBFFT_SYNC() __syncthreads()\n#endif\n";
     for (char const *T : {"float", "double"}) {
         std::string t = T, t2 = t + "2";
         os << "static __device__ __forceinline__ " << t2 << " mk(" << t << " a, " << t
@@ -1528,14 +1564,14 @@ void generate_fft1d_body(std::ostream &os, fft1d_configuration const &cfg,
 
 void generate_fft1d_body_aliased(std::ostream &os, fft1d_configuration const &cfg,
                                  std::string const &name, std::size_t group_size, bool alias_in,
-                                 bool alias_out) {
+                                 bool alias_out, bool lds_tables) {
     if (cfg.load_function || cfg.store_function) {
         throw bad_configuration("callbacks are not supported in fused kernels");
     }
     if (cfg.flags & (fft1d_input_f16 | fft1d_input_bf16 | fft1d_mfma)) {
         throw bad_configuration("half-precision input is not supported in fused kernels");
     }
-    kernel_gen g(cfg, name, group_size, alias_in, alias_out);
+    kernel_gen g(cfg, name, group_size, alias_in, alias_out, lds_tables);
     os << g.generate_body();
 }
 

@@ -17,10 +17,11 @@ std::string hip_prelude();
 void generate_fft1d_body(std::ostream &os, fft1d_configuration const &cfg,
                          std::string const &name, std::size_t group_size);
 /// Tile body whose input (alias_in) or output (alias_out) pointer may point into the LDS
-/// scratch passed to the body (real transforms included).
+/// scratch passed to the body (real transforms included). lds_tables: the body reads its
+/// twiddles from LDS copies that `name_tables(lid, n)` fills (emitted with the body).
 void generate_fft1d_body_aliased(std::ostream &os, fft1d_configuration const &cfg,
                                  std::string const &name, std::size_t group_size, bool alias_in,
-                                 bool alias_out);
+                                 bool alias_out, bool lds_tables = false);
 
 } // namespace bbfft
 

@@ -99,6 +99,25 @@ bool one_tile(configuration p, std::size_t kcols, device_info const &info, unsig
     return true;
 }
 
+/// Complex entries of the twiddle tables of pass c (stages 1.. and the real pre/post-pass).
+std::size_t twiddle_entries(fft1d_configuration const &c) {
+    std::size_t n = 0, p = c.factorization.empty() ? 1 : c.factorization[0];
+    for (std::size_t s = 1; s < c.factorization.size(); ++s) {
+        n += p * c.factorization[s];
+        p *= c.factorization[s];
+    }
+    if (c.type != transform_type::c2c && c.N % 2 == 0) {
+        n += c.N / 4 + 1;
+    }
+    return n;
+}
+
+/// Value of environment variable `name` ("" when unset).
+std::string env(char const *name) {
+    char const *v = std::getenv(name);
+    return v ? std::string(v) : std::string();
+}
+
 } // namespace
 
 std::string fused2_configuration::identifier() const {
@@ -108,13 +127,16 @@ std::string fused2_configuration::identifier() const {
         << fnv1a64(a.identifier() + ";" + b.identifier() + ";" + std::to_string(ka) + ";" +
                    std::to_string(kb) + ";" + std::to_string(plane_elems) + ";" +
                    std::to_string(in_plane_stride) + ";" + std::to_string(out_plane_stride) +
-                   ";" + std::to_string(stage_words));
+                   ";" + std::to_string(stage_words) + ";" + std::to_string(stage_load_bytes) +
+                   ";" + std::to_string(lds_barriers) + ";" + std::to_string(lds_tables) +
+                   ";" + std::to_string(waves_per_eu));
     return oss.str();
 }
 
 std::size_t fused2_configuration::lds_bytes() const {
     std::size_t const cb = 2 * static_cast<std::size_t>(a.fp);
-    return std::max({plane_elems * cb, a.lds_bytes(), b.lds_bytes()});
+    std::size_t const tables = lds_tables ? (twiddle_entries(a) + twiddle_entries(b)) * cb : 0;
+    return std::max({plane_elems * cb, a.lds_bytes(), b.lds_bytes()}) + tables;
 }
 
 bool configure_fused2(configuration const &pa, configuration const &pb, device_info const &info,
@@ -157,6 +179,15 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
     fc.in_plane_stride = ka * pa.istride[2];
     fc.out_plane_stride = kb * pb.ostride[2];
     fc.group_size = std::max(fc.a.work_group_size(), fc.b.work_group_size());
+    // Barriers and tables (BBFFT_FUSE_SYNC=full|lds, BBFFT_FUSE_TABLES=const|lds)
+    fc.lds_barriers = env("BBFFT_FUSE_SYNC") == "lds";
+    fc.lds_tables = env("BBFFT_FUSE_TABLES") == "lds";
+    if (!env("BBFFT_FUSE_WAVES").empty()) {
+        fc.waves_per_eu = static_cast<unsigned>(std::atoi(env("BBFFT_FUSE_WAVES").c_str()));
+    }
+    if (fc.lds_bytes() > info.local_memory_size) {
+        fc.lds_tables = false;
+    }
     if (fc.lds_bytes() > info.local_memory_size) {
         return false;
     }
@@ -183,6 +214,11 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
             st.a.work_group_size() == fc.a.work_group_size() &&
             st.lds_bytes() + 4 * words <= info.local_memory_size) {
             st.stage_words = words;
+            // whole 1 KB blocks per wave when the plane's input is a multiple of 16 bytes
+            // (BBFFT_FUSE_DMA=4 keeps one word per work-item)
+            std::size_t const W = fc.in_plane_stride * in_bytes / 4;
+            st.stage_load_bytes =
+                W % 4 == 0 && fc.group_size >= 64 && env("BBFFT_FUSE_DMA") == "16" ? 16 : 4;
             fc = st;
         }
     }
@@ -200,6 +236,17 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
     std::size_t const lds_elems = fc.lds_bytes() / (2 * static_cast<std::size_t>(fc.a.fp));
     // body names derive from the kernel name: several kernels may share one source
     std::string const fa = n + "_a", fb = n + "_b";
+    if (fc.lds_barriers) {
+        // Barrier ordering LDS only: waits for this wave's outstanding LDS operations, not for
+        // global loads and stores, nor for the direct global-to-LDS staging loads (a fence on
+        // LDS would wait for those, as they write LDS; the staging buffer is only read after
+        // BBFFT_WAIT_ALL). The memory clobber keeps the compiler from moving LDS accesses
+        // across it.
+        os << "#ifndef BBFFT_SYNC\n"
+              "#define BBFFT_SYNC() asm volatile(\"s_waitcnt lgkmcnt(0)\\n\\ts_barrier\" ::: "
+              "\"memory\")\n"
+              "#endif\n";
+    }
     os << hip_prelude();
     if (fc.stage_words) {
         // direct global-to-LDS load of one 32-bit word per work-item (the LDS address is the
@@ -210,20 +257,36 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
               "(__attribute__((address_space(3))) void*)(dst), 4, 0, 0)\n"
               "#define BBFFT_WAIT_ALL() __builtin_amdgcn_s_waitcnt(0)\n"
               "#endif\n";
+        if (fc.stage_load_bytes == 16) {
+            os << "#ifndef BBFFT_LDS_DMA16\n"
+                  "#define BBFFT_LDS_DMA16(src, dst) __builtin_amdgcn_global_load_lds("
+                  "(const __attribute__((address_space(1))) void*)(src), "
+                  "(__attribute__((address_space(3))) void*)(dst), 16, 0, 0)\n"
+                  "#endif\n";
+        }
     }
-    generate_fft1d_body_aliased(os, fc.a, fa, fc.group_size, false, true);
-    generate_fft1d_body_aliased(os, fc.b, fb, fc.group_size, true, false);
-    os << "extern \"C\" __global__ void __launch_bounds__(" << fc.group_size << ") " << n
+    generate_fft1d_body_aliased(os, fc.a, fa, fc.group_size, false, true, fc.lds_tables);
+    generate_fft1d_body_aliased(os, fc.b, fb, fc.group_size, true, false, fc.lds_tables);
+    os << "extern \"C\" __global__ void __launch_bounds__(" << fc.group_size << ") ";
+    if (fc.waves_per_eu) {
+        os << "__attribute__((amdgpu_waves_per_eu(" << fc.waves_per_eu << "))) ";
+    }
+    os << n
        << "(const " << in_t << "* in, " << out_t << "* out, unsigned long planes) {\n"
        << "  __shared__ " << T2 << " lds_buf[" << lds_elems << "];\n";
+    if (fc.lds_tables) {
+        // the first barrier of the plane loop orders these writes before any table read
+        os << "  " << fa << "_tables(threadIdx.x, " << fc.group_size << "u);\n"
+           << "  " << fb << "_tables(threadIdx.x, " << fc.group_size << "u);\n";
+    }
     if (!fc.stage_words) {
         os << "  for (unsigned long f = blockIdx.x; f < planes; f += gridDim.x) {\n"
+           << "    BBFFT_SYNC();\n"
            << "    " << fa << "(in + f * " << fc.in_plane_stride << "ul, lds_buf, " << fc.ka
            << "ul, 0ul, lds_buf, threadIdx.x);\n"
-           << "    __syncthreads();\n"
+           << "    BBFFT_SYNC();\n"
            << "    " << fb << "(lds_buf, out + f * " << fc.out_plane_stride << "ul, " << fc.kb
            << "ul, 0ul, lds_buf, threadIdx.x);\n"
-           << "    __syncthreads();\n"
            << "  }\n"
            << "}\n";
         return;
@@ -245,8 +308,20 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
        << "  const unsigned wv = threadIdx.x / 64u, ln = threadIdx.x % 64u;\n"
        << "  auto stage_copy = [&](unsigned long f) {\n"
        << "    const float* src = (const float*)in + f * " << W << "ul;\n"
-       << "    if (wv >= " << NWc << "u) return; // a partial last wave copies nothing\n"
-       << "    for (unsigned b = wv; b < " << NB << "u; b += " << NWc << "u) {\n"
+       << "    if (wv >= " << NWc << "u) return; // a partial last wave copies nothing\n";
+    if (fc.stage_load_bytes == 16) {
+        // 16 bytes per work-item: wave block b of 256 words lands at stage + 256 b, lane l's
+        // four words at + 4 l (the input pointer's alignment is uniform: one branch)
+        std::size_t const NB16 = (W + 255) / 256;
+        os << "    if ((reinterpret_cast<unsigned long>(in) & 15ul) == 0ul) {\n"
+           << "      for (unsigned b = wv; b < " << NB16 << "u; b += " << NWc << "u) {\n"
+           << "        const unsigned i = b * 256u + 4u * ln;\n"
+           << "        if (i < " << W << "u) BBFFT_LDS_DMA16(src + i, stage + b * 256u);\n"
+           << "      }\n"
+           << "      return;\n"
+           << "    }\n";
+    }
+    os << "    for (unsigned b = wv; b < " << NB << "u; b += " << NWc << "u) {\n"
        << "      const unsigned i = b * " << BS << "u + ln;\n"
        << "      if (i < " << W << "u) BBFFT_LDS_DMA4(src + i, stage + b * " << BS << "u);\n"
        << "    }\n"
@@ -255,14 +330,13 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
        << "  if (f < planes) stage_copy(f);\n"
        << "  for (; f < planes; f += gridDim.x) {\n"
        << "    BBFFT_WAIT_ALL();\n"
-       << "    __syncthreads();\n"
+       << "    BBFFT_SYNC();\n"
        << "    " << fa << "((const " << in_t << "*)stage, lds_buf, " << fc.ka
        << "ul, 0ul, lds_buf, threadIdx.x);\n"
-       << "    __syncthreads();\n"
+       << "    BBFFT_SYNC();\n"
        << "    if (f + gridDim.x < planes) stage_copy(f + gridDim.x);\n"
        << "    " << fb << "(lds_buf, out + f * " << fc.out_plane_stride << "ul, " << fc.kb
        << "ul, 0ul, lds_buf, threadIdx.x);\n"
-       << "    __syncthreads();\n"
        << "  }\n"
        << "}\n";
 }

@@ -164,6 +164,19 @@ struct BBFFT_EXPORT fused2_configuration {
     /// an LDS staging buffer with direct global-to-LDS loads while it still works on the
     /// previous plane (persistent grid); 0: pass a reads global memory itself.
     std::size_t stage_words = 0;
+    /// Bytes per work-item of each direct global-to-LDS staging load: 16 (whole 1 KB blocks
+    /// per wave) or 4. 16-byte loads need a 16-byte aligned input, which the kernel checks;
+    /// otherwise it takes the 4-byte loads.
+    unsigned stage_load_bytes = 4;
+    /// Barriers order LDS accesses only: each work-item's global accesses within a plane are
+    /// its own, so waits for outstanding global loads and stores are not needed at barriers.
+    bool lds_barriers = false;
+    /// Twiddle tables copied into LDS once per work-group (no global loads inside the plane
+    /// loop besides the staging loads).
+    bool lds_tables = false;
+    /// Occupancy target of the register allocator (waves per SIMD, 0: the compiler's default);
+    /// 8 lets two 1024-work-item groups share a CU when their LDS fits.
+    unsigned waves_per_eu = 0;
 
     std::string identifier() const;
     std::size_t lds_bytes() const; ///< plane and stage scratch (without the staging buffer)

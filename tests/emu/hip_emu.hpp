@@ -33,7 +33,14 @@ struct emu_dim3 {
 #define __builtin_nontemporal_load(p) (*(p))
 // direct global-to-LDS load: the work-item's word lands at the wave's LDS base + its lane
 #define BBFFT_LDS_DMA4(src, dst) (((float*)(dst))[threadIdx.x % 64u] = *(const float*)(src))
+// 16-byte form: the work-item's four words land at the wave's LDS base + 16 bytes x lane
+#define BBFFT_LDS_DMA16(src, dst)                                                               \
+    do {                                                                                        \
+        for (int q_ = 0; q_ < 4; ++q_)                                                          \
+            ((float *)(dst))[4u * (threadIdx.x % 64u) + q_] = ((const float *)(src))[q_];      \
+    } while (0)
 #define BBFFT_WAIT_ALL() ((void)0)
+#define BBFFT_SYNC() __syncthreads()
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))
 
 namespace emu {

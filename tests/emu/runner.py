@@ -22,7 +22,8 @@ def _build_so(tu):
     h = hashlib.sha1(tu.encode() + shim).hexdigest()[:16]
     so = os.path.join(_BUILD, f"k_{h}.so")
     if not os.path.exists(so):
-        src = os.path.join(_BUILD, f"k_{h}.cpp")
+        # per-process source and output names: parallel test workers may build the same unit
+        src = os.path.join(_BUILD, f"k_{h}.{os.getpid()}.cpp")
         with open(src, "w") as f:
             f.write(tu)
         tmp = so + f".tmp{os.getpid()}"
@@ -31,6 +32,7 @@ def _build_so(tu):
         subprocess.run([cxx, "-O1", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-w",
                         "-I", _HERE, src, "-o", tmp], check=True)
         os.replace(tmp, so)
+        os.remove(src)
     return ctypes.CDLL(so)
 
 

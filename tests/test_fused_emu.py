@@ -93,6 +93,47 @@ def test_fused_staged_matches_unstaged(t, M, dims, K, monkeypatch):
     assert np.array_equal(ys[0], ys[1])
 
 
+VARIANTS = [
+    {"BBFFT_FUSE_DMA": "16", "BBFFT_FUSE_SYNC": "lds", "BBFFT_FUSE_TABLES": "lds"},
+    {"BBFFT_FUSE_DMA": "16"},
+    {"BBFFT_FUSE_TABLES": "lds"},
+    {"BBFFT_FUSE_STAGE": "off", "BBFFT_FUSE_SYNC": "lds", "BBFFT_FUSE_TABLES": "lds",
+     "BBFFT_FUSE_WAVES": "8"},
+]
+
+
+@pytest.mark.parametrize("variant", range(len(VARIANTS)))
+@pytest.mark.parametrize("t,M,dims,K", [(1, 1, [32, 24], 5), (2, 1, [32, 24], 5),
+                                        (0, 2, [16, 12, 5], 3), (2, 1, [32, 16, 4], 3)])
+def test_fused_variants_match_default(t, M, dims, K, variant, monkeypatch):
+    """The fused kernel's variants (16-byte staging loads, barriers ordering LDS only,
+    twiddle tables in LDS, occupancy target) compute the same bits as the default kernel."""
+    cfg = _cfg(M, dims, K, 4, t)
+    shape = cfg.shape
+    D = len(dims)
+    half = list(shape)
+    half[1] = dims[0] // 2 + 1
+    ishape = half if t == 2 else shape
+    oshape = half if t == 1 else shape
+    n_in = oracle.extent(D, ishape, cfg.istride)
+    n_out = oracle.extent(D, oshape, cfg.ostride)
+    x = oracle.uniform_real(4, n_in) if t == 1 else oracle.uniform_complex(4, n_in)
+    x = x.astype(np.float32 if t == 1 else np.complex64)
+    base_name = bb.fused_configuration(cfg)["name"]
+    y0 = _run_plan(cfg, x, n_out)
+    for k, v in VARIANTS[variant].items():
+        monkeypatch.setenv(k, v)
+    fc = bb.fused_configuration(cfg)
+    src = bb.generate_fused_source(cfg)
+    assert fc["name"] != base_name
+    if VARIANTS[variant].get("BBFFT_FUSE_DMA") == "16":
+        assert "BBFFT_LDS_DMA16(" in src
+    if VARIANTS[variant].get("BBFFT_FUSE_TABLES") == "lds":
+        assert "_tables(threadIdx.x" in src
+    y1 = _run_plan(cfg, x, n_out)
+    assert np.array_equal(y0, y1)
+
+
 def test_fused_in_place_layout_r2c():
     """In-place r2c layout (padded real rows) read by the fused kernel."""
     cfg = _cfg(1, [16, 8, 4], 2, 4, 1, inplace=True)
