@@ -18,6 +18,7 @@
 #include "../base/math.hpp"
 #include "bbfft/bad_configuration.hpp"
 #include "bbfft/detail/generator_impl.hpp"
+#include "emitter.hpp"
 #include "roots.hpp"
 
 #include <algorithm>
@@ -318,9 +319,10 @@ class kernel_gen {
     /// work-group's LDS scratch (fused multi-dimensional kernels), so reads of the input and
     /// writes of the output are separated from scratch writes / reads by barriers.
     kernel_gen(fft1d_configuration const &c, std::string name, std::size_t group_size = 0,
-               bool alias_in = false, bool alias_out = false, bool lds_tables = false)
+               bool alias_in = false, bool alias_out = false, body_options const &opt = {})
         : c_(c), f_{c.fp}, name_(std::move(name)), group_size_(group_size), alias_in_(alias_in),
-          alias_out_(alias_out), lds_tables_(lds_tables) {
+          alias_out_(alias_out), lds_tables_(opt.lds_tables), lds_barriers_(opt.lds_barriers),
+          extra_param_(opt.extra_param), after_input_(opt.after_input) {
         nfft_ = c.N_fft();
         L_ = c.factorization.size();
         real_ = c.type != transform_type::c2c;
@@ -439,8 +441,9 @@ class kernel_gen {
         return "tout[" + off + "] = " + val + ";";
     }
 
-    /// Work-group barrier (BBFFT_SYNC: __syncthreads() unless the kernel's source defines it).
-    static std::string sync() { return "BBFFT_SYNC();"; }
+    /// Work-group barrier: __syncthreads(), or in fused kernels with lds_barriers a barrier
+    /// that orders LDS accesses only (BBFFT_LDS_SYNC, defined by the fused kernel's source).
+    std::string sync() const { return lds_barriers_ ? "BBFFT_LDS_SYNC();" : "__syncthreads();"; }
     /// Twiddle tables: __constant__ arrays, or their LDS copies (lds_tables_).
     std::string tw_name(std::size_t s) const {
         return name_ + "_tw" + std::to_string(s) + (lds_tables_ ? "_l" : "");
@@ -631,7 +634,8 @@ class kernel_gen {
     void emit_body() {
         w_.line("static __device__ __forceinline__ void " + name_ + "(const " + in_type() +
                 "* in, " + out_type() + "* out, unsigned long K, unsigned long tile, " + T2() +
-                "* lds_buf, unsigned lid) {");
+                "* lds_buf, unsigned lid" + (extra_param_.empty() ? "" : ", " + extra_param_) +
+                ") {");
         w_.open("");
         w_.line("const unsigned ml = lid % " + u(c_.Mb) + ";");
         w_.line("const unsigned t = (lid / " + u(c_.Mb) + ") % " + u(c_.Nt) + ";");
@@ -668,6 +672,13 @@ class kernel_gen {
                 emit_direct_stage(s);
             } else {
                 emit_stage(s);
+            }
+            if (s == 0 && !after_input_.empty()) {
+                // stage 0 ended with a barrier: every work-item's input reads are complete
+                if (L_ == 1 && c_.type != transform_type::r2c) {
+                    throw bad_configuration("after-input code needs a barrier after stage 0");
+                }
+                w_.line(after_input_);
             }
         }
         if (c_.type == transform_type::r2c) {
@@ -1482,7 +1493,10 @@ class kernel_gen {
     unsigned pad_shift_ = 0;
     bool real_ = false, half_ = false, pairs_ = false, use_lds_ = false;
     bool half_in_ = false, bf16_ = false, mfma_ = false;
-    bool lds_tables_ = false; // twiddles read from LDS copies (fused kernels)
+    bool lds_tables_ = false;   // twiddles read from LDS copies (fused kernels)
+    bool lds_barriers_ = false; // barriers order LDS only (fused kernels)
+    std::string extra_param_;  // extra parameter of a tile body
+    std::string after_input_;  // code run once stage 0 has read all of the tile's input
     int direct_ = 0; // radix of the direct DFT stage (0: none)
     bool local32_ = false;
     bool prefetch_ = false;
@@ -1494,7 +1508,6 @@ class kernel_gen {
 std::string hip_prelude() {
     std::ostringstream os;
     os << "#ifndef BBFFT_HIP_PRELUDE\n#define BBFFT_HIP_PRELUDE\n";
-    os << "#ifndef BBFFT_SYNC\n#define BBFFT_SYNC() __syncthreads()\n#endif\n";
     for (char const *T : {"float", "double"}) {
         std::string t = T, t2 = t + "2";
         os << "static __device__ __forceinline__ " << t2 << " mk(" << t << " a, " << t
@@ -1564,14 +1577,14 @@ void generate_fft1d_body(std::ostream &os, fft1d_configuration const &cfg,
 
 void generate_fft1d_body_aliased(std::ostream &os, fft1d_configuration const &cfg,
                                  std::string const &name, std::size_t group_size, bool alias_in,
-                                 bool alias_out, bool lds_tables) {
+                                 bool alias_out, body_options const &opt) {
     if (cfg.load_function || cfg.store_function) {
         throw bad_configuration("callbacks are not supported in fused kernels");
     }
     if (cfg.flags & (fft1d_input_f16 | fft1d_input_bf16 | fft1d_mfma)) {
         throw bad_configuration("half-precision input is not supported in fused kernels");
     }
-    kernel_gen g(cfg, name, group_size, alias_in, alias_out, lds_tables);
+    kernel_gen g(cfg, name, group_size, alias_in, alias_out, opt);
     os << g.generate_body();
 }
 

@@ -16,12 +16,23 @@ std::string hip_prelude();
 /// work-group of `group_size` work-items (>= cfg.work_group_size()).
 void generate_fft1d_body(std::ostream &os, fft1d_configuration const &cfg,
                          std::string const &name, std::size_t group_size);
+/// Options of a tile body in a fused kernel.
+struct body_options {
+    /// The body reads its twiddles from LDS copies that `name_tables(lid, n)` fills (emitted
+    /// with the body).
+    bool lds_tables = false;
+    /// Barriers order LDS accesses only (BBFFT_LDS_SYNC(), which the kernel's source defines).
+    bool lds_barriers = false;
+    /// Extra parameter appended to the body's signature (e.g. "const float* next").
+    std::string extra_param;
+    /// Statement emitted once stage 0 has read all of the tile's input (after its barrier).
+    std::string after_input;
+};
 /// Tile body whose input (alias_in) or output (alias_out) pointer may point into the LDS
-/// scratch passed to the body (real transforms included). lds_tables: the body reads its
-/// twiddles from LDS copies that `name_tables(lid, n)` fills (emitted with the body).
+/// scratch passed to the body (real transforms included).
 void generate_fft1d_body_aliased(std::ostream &os, fft1d_configuration const &cfg,
                                  std::string const &name, std::size_t group_size, bool alias_in,
-                                 bool alias_out, bool lds_tables = false);
+                                 bool alias_out, body_options const &opt = {});
 
 } // namespace bbfft
 

@@ -129,7 +129,8 @@ std::string fused2_configuration::identifier() const {
                    std::to_string(in_plane_stride) + ";" + std::to_string(out_plane_stride) +
                    ";" + std::to_string(stage_words) + ";" + std::to_string(stage_load_bytes) +
                    ";" + std::to_string(lds_barriers) + ";" + std::to_string(lds_tables) +
-                   ";" + std::to_string(waves_per_eu));
+                   ";" + std::to_string(waves_per_eu) + ";" + std::to_string(copy_wave0) + ";" +
+                   std::to_string(early_copy));
     return oss.str();
 }
 
@@ -179,9 +180,23 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
     fc.in_plane_stride = ka * pa.istride[2];
     fc.out_plane_stride = kb * pb.ostride[2];
     fc.group_size = std::max(fc.a.work_group_size(), fc.b.work_group_size());
-    // Barriers and tables (BBFFT_FUSE_SYNC=full|lds, BBFFT_FUSE_TABLES=const|lds)
-    fc.lds_barriers = env("BBFFT_FUSE_SYNC") == "lds";
-    fc.lds_tables = env("BBFFT_FUSE_TABLES") == "lds";
+    // Kernel variants (all compute the same bits). Measured on MI355X with bench.py's 3-D legs
+    // (32 x 128^3 fp32, whole transform, processes alternated, 3 rounds): r2c 0.215 ms with
+    // barriers ordering LDS only, LDS twiddle tables, 16-byte staging loads issued right after
+    // pass a's first stage by the waves idle in pass b, against 0.225 without the idle-wave
+    // split, 0.234 without the early copy, 0.234 with 4-byte loads and 0.232 with none of
+    // these; c2r (c2c first pass, all work-items busy in pass b) 0.229 ms with 16-byte loads
+    // alone, against 0.233 with 4-byte loads, 0.233-0.234 with LDS-only barriers and tables
+    // and 0.238 with the early copy as well. Environment overrides for A/B:
+    // BBFFT_FUSE_SYNC=full|lds, BBFFT_FUSE_TABLES=const|lds, BBFFT_FUSE_DMA=4|16,
+    // BBFFT_FUSE_EARLY=off|on, BBFFT_FUSE_DMAWAVES=all|split, BBFFT_FUSE_WAVES=<waves per SIMD>.
+    bool const r2c_first = pa.type == transform_type::r2c;
+    auto option = [](char const *name, char const *on, bool dflt) {
+        std::string const v = env(name);
+        return v.empty() ? dflt : v == on;
+    };
+    fc.lds_barriers = option("BBFFT_FUSE_SYNC", "lds", r2c_first);
+    fc.lds_tables = option("BBFFT_FUSE_TABLES", "lds", r2c_first);
     if (!env("BBFFT_FUSE_WAVES").empty()) {
         fc.waves_per_eu = static_cast<unsigned>(std::atoi(env("BBFFT_FUSE_WAVES").c_str()));
     }
@@ -217,8 +232,21 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
             // whole 1 KB blocks per wave when the plane's input is a multiple of 16 bytes
             // (BBFFT_FUSE_DMA=4 keeps one word per work-item)
             std::size_t const W = fc.in_plane_stride * in_bytes / 4;
-            st.stage_load_bytes =
-                W % 4 == 0 && fc.group_size >= 64 && env("BBFFT_FUSE_DMA") == "16" ? 16 : 4;
+            st.stage_load_bytes = W % 4 == 0 && fc.group_size >= 64 &&
+                                          option("BBFFT_FUSE_DMA", "16", true)
+                                      ? 16
+                                      : 4;
+            // staging loads from the waves idle in pass b (BBFFT_FUSE_DMAWAVES=split)
+            std::size_t const b_waves = (st.b.work_group_size() + 63) / 64;
+            if (option("BBFFT_FUSE_DMAWAVES", "split", r2c_first) && st.lds_barriers &&
+                st.lds_tables &&
+                st.group_size % 64 == 0 && b_waves < st.group_size / 64) {
+                st.copy_wave0 = static_cast<unsigned>(b_waves);
+            }
+            // next plane's copy issued right after pass a's first stage (BBFFT_FUSE_EARLY=on);
+            // only useful when barriers do not wait for it
+            st.early_copy = option("BBFFT_FUSE_EARLY", "on", r2c_first) &&
+                            (st.a.factorization.size() > 1 || st.a.type == transform_type::r2c);
             fc = st;
         }
     }
@@ -242,8 +270,11 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
         // LDS would wait for those, as they write LDS; the staging buffer is only read after
         // BBFFT_WAIT_ALL). The memory clobber keeps the compiler from moving LDS accesses
         // across it.
-        os << "#ifndef BBFFT_SYNC\n"
-              "#define BBFFT_SYNC() asm volatile(\"s_waitcnt lgkmcnt(0)\\n\\ts_barrier\" ::: "
+        // (Every kernel source that defines it defines it the same way, so kernels compiled
+        // together in one translation unit agree; kernels without lds_barriers use
+        // __syncthreads().)
+        os << "#ifndef BBFFT_LDS_SYNC\n"
+              "#define BBFFT_LDS_SYNC() asm volatile(\"s_waitcnt lgkmcnt(0)\\n\\ts_barrier\" ::: "
               "\"memory\")\n"
               "#endif\n";
     }
@@ -265,8 +296,56 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
                   "#endif\n";
         }
     }
-    generate_fft1d_body_aliased(os, fc.a, fa, fc.group_size, false, true, fc.lds_tables);
-    generate_fft1d_body_aliased(os, fc.b, fb, fc.group_size, true, false, fc.lds_tables);
+    std::size_t const in_bytes = fc.a.type == transform_type::r2c
+                                     ? static_cast<std::size_t>(fc.a.fp)
+                                     : 2 * static_cast<std::size_t>(fc.a.fp);
+    std::size_t const W = fc.in_plane_stride * in_bytes / 4; // words of one plane's input
+    std::string const stage = n + "_stage", copy = n + "_copy";
+    if (fc.stage_words) {
+        // Staged: plane f's raw input (W words) sits in `stage` before pass a starts; the copy
+        // of the work-group's next plane is issued once pass a no longer reads `stage` (after
+        // pass a, or with early_copy right after pass a's first stage) and overlaps the rest.
+        // Blocks of BS words, one per copying wave and step: lane l's word lands at the block
+        // base + l; a group smaller than one wave copies group_size words per step.
+        std::size_t const BS = std::min<std::size_t>(64, fc.group_size);
+        std::size_t const wv0 = fc.copy_wave0; // copying waves: wv0 .. wv0 + NWc - 1
+        std::size_t const NWc = fc.group_size >= 64 ? fc.group_size / 64 - wv0 : 1;
+        std::size_t const NB = (W + BS - 1) / BS;
+        os << "// copy of one plane's input (src, or nothing when src is null) into the staging\n"
+           << "// buffer `stage` (LDS)\n"
+           << "static __device__ __forceinline__ void " << copy
+           << "(const float* src, float* stage) {\n"
+           << "  const unsigned ln = threadIdx.x % 64u;\n"
+           << "  const unsigned wv = threadIdx.x / 64u - " << wv0 << "u; // copying wave index\n"
+           << "  if (!src || wv >= " << NWc << "u) return; // other waves copy nothing\n";
+        if (fc.stage_load_bytes == 16) {
+            // 16 bytes per work-item: wave block b of 256 words lands at stage + 256 b, lane
+            // l's four words at + 4 l (the pointer's alignment is uniform: one branch)
+            std::size_t const NB16 = (W + 255) / 256;
+            os << "  if ((reinterpret_cast<unsigned long>(src) & 15ul) == 0ul) {\n"
+               << "    for (unsigned b = wv; b < " << NB16 << "u; b += " << NWc << "u) {\n"
+               << "      const unsigned i = b * 256u + 4u * ln;\n"
+               << "      if (i < " << W << "u) BBFFT_LDS_DMA16(src + i, stage + b * 256u);\n"
+               << "    }\n"
+               << "    return;\n"
+               << "  }\n";
+        }
+        os << "  for (unsigned b = wv; b < " << NB << "u; b += " << NWc << "u) {\n"
+           << "    const unsigned i = b * " << BS << "u + ln;\n"
+           << "    if (i < " << W << "u) BBFFT_LDS_DMA4(src + i, stage + b * " << BS << "u);\n"
+           << "  }\n"
+           << "}\n";
+    }
+    body_options oa, ob;
+    oa.lds_tables = ob.lds_tables = fc.lds_tables;
+    oa.lds_barriers = ob.lds_barriers = fc.lds_barriers;
+    std::string const sync = fc.lds_barriers ? "BBFFT_LDS_SYNC();" : "__syncthreads();";
+    if (fc.stage_words && fc.early_copy) {
+        oa.extra_param = "const float* next_input, float* stage";
+        oa.after_input = copy + "(next_input, stage);";
+    }
+    generate_fft1d_body_aliased(os, fc.a, fa, fc.group_size, false, true, oa);
+    generate_fft1d_body_aliased(os, fc.b, fb, fc.group_size, true, false, ob);
     os << "extern \"C\" __global__ void __launch_bounds__(" << fc.group_size << ") ";
     if (fc.waves_per_eu) {
         os << "__attribute__((amdgpu_waves_per_eu(" << fc.waves_per_eu << "))) ";
@@ -281,62 +360,43 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
     }
     if (!fc.stage_words) {
         os << "  for (unsigned long f = blockIdx.x; f < planes; f += gridDim.x) {\n"
-           << "    BBFFT_SYNC();\n"
+           << "    " << sync << "\n"
            << "    " << fa << "(in + f * " << fc.in_plane_stride << "ul, lds_buf, " << fc.ka
            << "ul, 0ul, lds_buf, threadIdx.x);\n"
-           << "    BBFFT_SYNC();\n"
+           << "    " << sync << "\n"
            << "    " << fb << "(lds_buf, out + f * " << fc.out_plane_stride << "ul, " << fc.kb
            << "ul, 0ul, lds_buf, threadIdx.x);\n"
            << "  }\n"
            << "}\n";
         return;
     }
-    // Staged: plane f's raw input (W words) sits in `stage` before pass a starts; the copy of
-    // plane f + gridDim.x is issued once pass a has consumed `stage` and overlaps pass b.
-    // Each wave copies blocks of 64 words: lane l's word lands at the block base + l.
-    std::size_t const in_bytes = fc.a.type == transform_type::r2c
-                                     ? static_cast<std::size_t>(fc.a.fp)
-                                     : 2 * static_cast<std::size_t>(fc.a.fp);
-    std::size_t const W = fc.in_plane_stride * in_bytes / 4;
-    // Blocks of BS words, one per copying wave and step: the full waves copy 64 words each;
-    // a group smaller than one wave copies group_size words per step with its only wave.
-    std::size_t const BS = std::min<std::size_t>(64, fc.group_size);
-    std::size_t const NWc = fc.group_size >= 64 ? fc.group_size / 64 : 1;
-    std::size_t const NB = (W + BS - 1) / BS;
+    std::size_t const NWc = fc.group_size >= 64 ? fc.group_size / 64 - fc.copy_wave0 : 1;
+    std::string const next = "(f + gridDim.x < planes ? (const float*)in + (f + gridDim.x) * " +
+                             std::to_string(W) + "ul : nullptr)";
     // 16-byte aligned: pass a may read the staged input as float2 / double2
-    os << "  __shared__ __attribute__((aligned(16))) float stage[" << fc.stage_words << "];\n"
-       << "  const unsigned wv = threadIdx.x / 64u, ln = threadIdx.x % 64u;\n"
-       << "  auto stage_copy = [&](unsigned long f) {\n"
-       << "    const float* src = (const float*)in + f * " << W << "ul;\n"
-       << "    if (wv >= " << NWc << "u) return; // a partial last wave copies nothing\n";
-    if (fc.stage_load_bytes == 16) {
-        // 16 bytes per work-item: wave block b of 256 words lands at stage + 256 b, lane l's
-        // four words at + 4 l (the input pointer's alignment is uniform: one branch)
-        std::size_t const NB16 = (W + 255) / 256;
-        os << "    if ((reinterpret_cast<unsigned long>(in) & 15ul) == 0ul) {\n"
-           << "      for (unsigned b = wv; b < " << NB16 << "u; b += " << NWc << "u) {\n"
-           << "        const unsigned i = b * 256u + 4u * ln;\n"
-           << "        if (i < " << W << "u) BBFFT_LDS_DMA16(src + i, stage + b * 256u);\n"
-           << "      }\n"
-           << "      return;\n"
-           << "    }\n";
-    }
-    os << "    for (unsigned b = wv; b < " << NB << "u; b += " << NWc << "u) {\n"
-       << "      const unsigned i = b * " << BS << "u + ln;\n"
-       << "      if (i < " << W << "u) BBFFT_LDS_DMA4(src + i, stage + b * " << BS << "u);\n"
-       << "    }\n"
-       << "  };\n"
+    os << "  __shared__ __attribute__((aligned(16))) float " << stage << "[" << fc.stage_words
+       << "];\n"
        << "  unsigned long f = blockIdx.x;\n"
-       << "  if (f < planes) stage_copy(f);\n"
-       << "  for (; f < planes; f += gridDim.x) {\n"
-       << "    BBFFT_WAIT_ALL();\n"
-       << "    BBFFT_SYNC();\n"
-       << "    " << fa << "((const " << in_t << "*)stage, lds_buf, " << fc.ka
+       << "  if (f < planes) " << copy << "((const float*)in + f * " << W << "ul, " << stage
+       << ");\n"
+       << "  for (; f < planes; f += gridDim.x) {\n";
+    if (fc.copy_wave0) {
+        os << "    if (threadIdx.x / 64u - " << fc.copy_wave0 << "u < " << NWc
+           << "u) BBFFT_WAIT_ALL(); // copying waves\n";
+    } else {
+        os << "    BBFFT_WAIT_ALL();\n";
+    }
+    os << "    " << sync << "\n"
+       << "    " << fa << "((const " << in_t << "*)" << stage << ", lds_buf, " << fc.ka
+       << "ul, 0ul, lds_buf, threadIdx.x"
+       << (fc.early_copy ? ", " + next + ", " + stage : std::string()) << ");\n"
+       << "    " << sync << "\n";
+    if (!fc.early_copy) {
+        os << "    " << copy << "(" << next << ", " << stage << ");\n";
+    }
+    os << "    " << fb << "(lds_buf, out + f * " << fc.out_plane_stride << "ul, " << fc.kb
        << "ul, 0ul, lds_buf, threadIdx.x);\n"
-       << "    BBFFT_SYNC();\n"
-       << "    if (f + gridDim.x < planes) stage_copy(f + gridDim.x);\n"
-       << "    " << fb << "(lds_buf, out + f * " << fc.out_plane_stride << "ul, " << fc.kb
-       << "ul, 0ul, lds_buf, threadIdx.x);\n"
+       << "    " << sync << "\n"
        << "  }\n"
        << "}\n";
 }

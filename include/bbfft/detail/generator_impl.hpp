@@ -177,6 +177,14 @@ struct BBFFT_EXPORT fused2_configuration {
     /// Occupancy target of the register allocator (waves per SIMD, 0: the compiler's default);
     /// 8 lets two 1024-work-item groups share a CU when their LDS fits.
     unsigned waves_per_eu = 0;
+    /// First wave that issues the staging loads (0: all waves). With LDS-only barriers and LDS
+    /// tables, waves from this one on have no work in pass b and issue no global stores, so
+    /// waiting for their staging loads does not wait for the plane's stores (which share the
+    /// wait counter with the loads).
+    unsigned copy_wave0 = 0;
+    /// The next plane's staging loads start as soon as pass a's first stage has read the
+    /// staged input (instead of after pass a).
+    bool early_copy = false;
 
     std::string identifier() const;
     std::size_t lds_bytes() const; ///< plane and stage scratch (without the staging buffer)

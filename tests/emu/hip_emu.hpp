@@ -40,7 +40,7 @@ struct emu_dim3 {
             ((float *)(dst))[4u * (threadIdx.x % 64u) + q_] = ((const float *)(src))[q_];      \
     } while (0)
 #define BBFFT_WAIT_ALL() ((void)0)
-#define BBFFT_SYNC() __syncthreads()
+#define BBFFT_LDS_SYNC() __syncthreads()
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))
 
 namespace emu {

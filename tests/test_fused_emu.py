@@ -86,25 +86,36 @@ def test_fused_staged_matches_unstaged(t, M, dims, K, monkeypatch):
             monkeypatch.setenv("BBFFT_FUSE_STAGE", "off")
         fc = bb.fused_configuration(cfg)
         src = bb.generate_fused_source(cfg)
-        assert ("stage_copy" in src) == (stage == "on")
+        assert ("_stage[" in src) == (stage == "on")
         oshape = half if t == 1 else shape
         n_out = oracle.extent(D, oshape, cfg.ostride)
         ys.append(_run_plan(cfg, x, n_out))
     assert np.array_equal(ys[0], ys[1])
 
 
+# every variant setting off; VARIANTS override it (an empty dict: the plan's defaults)
+PLAIN = {"BBFFT_FUSE_DMA": "4", "BBFFT_FUSE_SYNC": "full", "BBFFT_FUSE_TABLES": "const",
+         "BBFFT_FUSE_EARLY": "off", "BBFFT_FUSE_DMAWAVES": "all"}
 VARIANTS = [
+    None,
     {"BBFFT_FUSE_DMA": "16", "BBFFT_FUSE_SYNC": "lds", "BBFFT_FUSE_TABLES": "lds"},
     {"BBFFT_FUSE_DMA": "16"},
     {"BBFFT_FUSE_TABLES": "lds"},
     {"BBFFT_FUSE_STAGE": "off", "BBFFT_FUSE_SYNC": "lds", "BBFFT_FUSE_TABLES": "lds",
      "BBFFT_FUSE_WAVES": "8"},
+    {"BBFFT_FUSE_DMA": "16", "BBFFT_FUSE_SYNC": "lds", "BBFFT_FUSE_TABLES": "lds",
+     "BBFFT_FUSE_DMAWAVES": "split"},
+    {"BBFFT_FUSE_DMA": "16", "BBFFT_FUSE_SYNC": "lds", "BBFFT_FUSE_TABLES": "lds",
+     "BBFFT_FUSE_EARLY": "on"},
+    {"BBFFT_FUSE_SYNC": "lds", "BBFFT_FUSE_TABLES": "lds", "BBFFT_FUSE_EARLY": "on",
+     "BBFFT_FUSE_DMAWAVES": "split"},
 ]
 
 
 @pytest.mark.parametrize("variant", range(len(VARIANTS)))
 @pytest.mark.parametrize("t,M,dims,K", [(1, 1, [32, 24], 5), (2, 1, [32, 24], 5),
-                                        (0, 2, [16, 12, 5], 3), (2, 1, [32, 16, 4], 3)])
+                                        (0, 2, [16, 12, 5], 3), (2, 1, [32, 16, 4], 3),
+                                        (1, 1, [16, 64], 9), (0, 1, [8, 32, 3], 5)])
 def test_fused_variants_match_default(t, M, dims, K, variant, monkeypatch):
     """The fused kernel's variants (16-byte staging loads, barriers ordering LDS only,
     twiddle tables in LDS, occupancy target) compute the same bits as the default kernel."""
@@ -119,17 +130,29 @@ def test_fused_variants_match_default(t, M, dims, K, variant, monkeypatch):
     n_out = oracle.extent(D, oshape, cfg.ostride)
     x = oracle.uniform_real(4, n_in) if t == 1 else oracle.uniform_complex(4, n_in)
     x = x.astype(np.float32 if t == 1 else np.complex64)
+    for k, v in PLAIN.items():
+        monkeypatch.setenv(k, v)
     base_name = bb.fused_configuration(cfg)["name"]
     y0 = _run_plan(cfg, x, n_out)
-    for k, v in VARIANTS[variant].items():
+    var = VARIANTS[variant]
+    if var is None:  # the defaults
+        for k in PLAIN:
+            monkeypatch.delenv(k)
+        var = {}
+    for k, v in var.items():
         monkeypatch.setenv(k, v)
     fc = bb.fused_configuration(cfg)
     src = bb.generate_fused_source(cfg)
-    assert fc["name"] != base_name
-    if VARIANTS[variant].get("BBFFT_FUSE_DMA") == "16":
+    if VARIANTS[variant] is not None or t == 1:
+        assert fc["name"] != base_name
+    if var.get("BBFFT_FUSE_DMA") == "16":
         assert "BBFFT_LDS_DMA16(" in src
-    if VARIANTS[variant].get("BBFFT_FUSE_TABLES") == "lds":
+    if var.get("BBFFT_FUSE_TABLES") == "lds":
         assert "_tables(threadIdx.x" in src
+    if var.get("BBFFT_FUSE_DMAWAVES") == "split" and dims in ([16, 64], [8, 32, 3]):
+        assert "copying waves\n" in src
+    if var.get("BBFFT_FUSE_EARLY") == "on":
+        assert "(next_input, stage);" in src
     y1 = _run_plan(cfg, x, n_out)
     assert np.array_equal(y0, y1)
 
@@ -180,3 +203,24 @@ def test_fusion_policy(monkeypatch):
     assert bb.fused_configuration(big)["lds_bytes"] > 96 * 1024
     monkeypatch.setenv("BBFFT_FUSE", "off")
     assert bb.fused_configuration(bb.parse_fft_descriptor("scfo64x64*100")) is None
+
+
+def test_macro_definitions_agree_across_kernels():
+    """Ahead-of-time archives compile many kernels in one translation unit, where the first
+    definition of an include-guarded macro wins: every macro that generated sources define must
+    therefore be defined identically by all of them (kernel-specific behaviour never goes
+    through a macro)."""
+    import re
+    srcs = [bb.generate_fused_source(bb.parse_fft_descriptor(d))
+            for d in ("srfo128x128x128*32", "srbo128x128x128*32", "scfo64x64x64*8",
+                      "srfo1.16x64*300")]
+    for d in ("scfo8.256*32768", "srfo3.128*33", "srbo5.96*33", "dcbo8.60*5", "scfo1.509*33"):
+        cfg = bb.parse_fft_descriptor(d)
+        srcs.append(bb.generate_source(cfg, 0, bb.GFX950_INFO))
+    defs = {}
+    for src in srcs:
+        for m in re.finditer(r"^#define\s+(\w+)(.*)$", src, re.M):
+            defs.setdefault(m.group(1), set()).add(m.group(2).strip())
+    assert "BBFFT_LDS_SYNC" in defs
+    for name, bodies in defs.items():
+        assert len(bodies) == 1, (name, bodies)

@@ -218,3 +218,35 @@ def test_fused_staged_vs_unstaged(desc, monkeypatch):
                                                       1j * rng.uniform(-1, 1, n_in))
         ys.append(execute(cfg, x, plan=plan))
     assert np.array_equal(ys[0].view(np.uint8), ys[1].view(np.uint8))
+
+
+# every fused-kernel variant setting off (the plan defaults switch some of them on)
+FUSE_PLAIN = {"BBFFT_FUSE_DMA": "4", "BBFFT_FUSE_SYNC": "full", "BBFFT_FUSE_TABLES": "const",
+              "BBFFT_FUSE_EARLY": "off", "BBFFT_FUSE_DMAWAVES": "all"}
+
+
+@pytest.mark.parametrize("desc", ["srfo128x128x128*4", "srbo128x128x128*4", "srfo1.16x64*300",
+                                  "scfo64x64x64*8", "drfo48x40x8*400"])
+def test_fused_default_variant_vs_plain(desc, monkeypatch):
+    """The plan's default fused kernel (for r2c plans: barriers ordering LDS only, LDS twiddle
+    tables, 16-byte staging loads issued after pass a's first stage by the waves idle in pass
+    b) gives the same bits as the fused kernel with all of these switched off."""
+    cfg = bb.parse_fft_descriptor(desc)
+    monkeypatch.setenv("BBFFT_FUSE", "on")
+    rng = np.random.default_rng(11)
+    ishape, _ = io_shapes(cfg)
+    n_in = oracle.extent(cfg.dim, ishape, cfg.istride)
+    t = int(cfg.type)
+    x = rng.uniform(-1, 1, n_in) if t == 1 else (rng.uniform(-1, 1, n_in) +
+                                                  1j * rng.uniform(-1, 1, n_in))
+    ys = []
+    for plain in (True, False):
+        for k, v in FUSE_PLAIN.items():
+            if plain:
+                monkeypatch.setenv(k, v)
+            else:
+                monkeypatch.delenv(k, raising=False)
+        plan = bb.make_plan(cfg, cache=bb.default_cache())
+        assert any(l["kernel"].startswith("bbfft_fused2") for l in plan.launches())
+        ys.append(execute(cfg, x, plan=plan))
+    assert np.array_equal(ys[0].view(np.uint8), ys[1].view(np.uint8))
