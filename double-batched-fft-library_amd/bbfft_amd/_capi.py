@@ -116,6 +116,18 @@ def lib_path():
     return os.environ.get("BBFFT_LIB", DEFAULT_LIB)
 
 
+def _init_device_runtime():
+    """Let torch (if present) initialise the HIP runtime before the library is loaded: the
+    library links the system HIP runtime, and a process that loaded it before torch touched
+    the GPU found no device on the first torch GPU call (hipErrorNoDevice on MI355X). torch
+    initialising first makes both use one runtime, as in every torch-first program."""
+    try:
+        import torch
+    except ImportError:  # pragma: no cover - the C ABI works without torch
+        return
+    torch.cuda.is_available()
+
+
 def load():
     """Load libbbfft_hip.so (once) and declare the prototypes."""
     global _lib
@@ -126,6 +138,7 @@ def load():
         raise ImportError(
             f"libbbfft_hip.so not found at {path}; build it with __graft_entry__.build() "
             "or `make -C double-batched-fft-library_amd`")
+    _init_device_runtime()
     L = ctypes.CDLL(path)
     c_int, c_uint, c_size, c_char_p, c_void_p = (ctypes.c_int, ctypes.c_uint, ctypes.c_size_t,
                                                  ctypes.c_char_p, ctypes.c_void_p)

@@ -4,6 +4,7 @@ hints for each (precision, transform type, N, M), and writes the table the libra
 (double-batched-fft-library_amd/tuning/gfx950.inc).
 
     python -m bbfft_amd.autotune [-o TABLE] [--rounds 2] DESCRIPTOR...
+    python -m bbfft_amd.autotune [-o TABLE] --merge --ragged-serving
 
 Search: coordinate descent from the library's heuristic default, one parameter group at a time
 (factorisation + work-items, K tile, M tile, LDS padding, streaming hints, occupancy target,
@@ -320,6 +321,103 @@ def merge_table(path, new_lines):
     return [ln for ln in old if _key(ln) not in keys] + list(new_lines)
 
 
+RAGGED_GROUP = 256  # work-items of the persistent ragged kernel's work-group
+RAGGED_WAVES = 5  # its occupancy target (waves per SIMD), shared by all variants
+
+
+def ragged_candidates(nfft, M, rmax=16, group=RAGGED_GROUP):
+    """(factorisation, Nt) pairs for a ragged variant: up to three stages, radices <= rmax,
+    one transform's work-items times M within the common work-group."""
+    out = []
+    for f in candidate_factorizations(nfft, 4, per_length=3):
+        if len(f) > 3 or max(f) > rmax:
+            continue
+        nt = nfft // max(f)
+        if M * nt <= group:
+            out.append((f, nt))
+    return out
+
+
+def _ragged_tuning(f, nt, M, pad, flags):
+    return {"factorization": list(f), "Nt": nt, "Mb": M, "Kb": max(1, RAGGED_GROUP // (M * nt)),
+            "lds_pad": pad, "no_pad": pad == 0, "flags": flags,
+            "waves_per_eu": RAGGED_WAVES, "groups_per_cu": 0}
+
+
+def tune_ragged_serving(reps=20, confirm_rounds=4, confirm_reps=50, elements=1 << 25,
+                        workers=16, verbose=True):
+    """Tune the variants (N, M) of the seeded ragged serving workload
+    (bbfft_amd.ragged.synthetic_serving_jobs) under the persistent kernel's constraints: each
+    candidate (factorisation with radices <= 16, work-items per transform, LDS padding, access
+    hints) runs as a plain kernel with the ragged kernel's work-group size and occupancy target
+    on `elements` complex elements, so register pressure that would spill in the shared kernel
+    shows as a slow candidate. Direction does not change the cost, so forward and backward share
+    an entry. Candidates are compiled in parallel first. Returns the table lines
+    ("f32 c2c-ragged N M : ...") ordered by the variants' share of the workload."""
+    from concurrent.futures import ThreadPoolExecutor
+    from . import ragged
+    jobs = ragged.synthetic_serving_jobs()
+    var = ragged.job_variants(jobs)
+    weight, current = {}, {}
+    for j, v in zip(jobs, var):
+        weight[(j.N, j.M)] = weight.get((j.N, j.M), 0) + j.elements
+        if (j.N, j.M) not in current:
+            current[(j.N, j.M)] = ragged.variant_configuration(jobs, v)
+    total = sum(weight.values())
+    lines = []
+    for (N, M), w in sorted(weight.items(), key=lambda kv: -kv[1]):
+        t0 = time.time()
+        K = max(64, elements // (M * N))
+        desc = f"scfo{M}.{N}*{K}"
+        tuner = Tuner(desc, reps=reps, verbose=verbose)
+        cur = current[(N, M)]
+        base = _ragged_tuning(cur["factorization"], cur["Nt"], cur["Mb"], cur["lds_pad"], 0)
+        base["Kb"] = cur["Kb"]
+        # reference output: the library's plain plan for the shape
+        plan = make_plan(tuner.cfg, cache=tuner.cache)
+        plan.execute(tuner.x, tuner.ref)
+        tuner.torch.cuda.synchronize()
+        cands = [_ragged_tuning(f, nt, M, 16, 0) for f, nt in ragged_candidates(tuner.nfft, M)]
+        with ThreadPoolExecutor(max_workers=workers) as ex:  # compile (hiprtc) in parallel
+            list(ex.map(lambda t: _compile_quietly(tuner, t), [base] + cands))
+        tuner.default_ms = tuner.evaluate(base)
+        for t in cands:
+            tuner.evaluate(t)
+        # padding and access hints for the two fastest factorisations
+        top = sorted((ms, k) for k, ms in tuner.results.items() if ms < float("inf"))[:2]
+        more = []
+        for _, k in top:
+            t = json.loads(k)
+            for pad in (0, 16):
+                for fl in (0, 1, 2, 3):
+                    more.append(_ragged_tuning(t["factorization"], t["Nt"], M, pad, fl))
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            list(ex.map(lambda t: _compile_quietly(tuner, t), more))
+        for t in more:
+            tuner.evaluate(t)
+        best, best_ms = tuner.confirm(rounds=confirm_rounds, reps=confirm_reps)
+        fac = "x".join(str(f) for f in best["factorization"])
+        line = (f"f32 c2c-ragged {N} {M} : {fac} {best['Nt']} {best['Mb']} {best['Kb']} "
+                f"{best['lds_pad']} {best['flags'] & 3} {RAGGED_WAVES}   # "
+                f"{tuner.bytes / tuner.default_ms / 1e6:.0f} -> {tuner.bytes / best_ms / 1e6:.0f}"
+                f" GB/s, {desc} as ragged variant ({100.0 * w / total:.1f} % of the serving "
+                f"workload)")
+        lines.append(line)
+        print(json.dumps({"variant": [N, M], "best": best, "ms": round(best_ms, 5),
+                          "current_ms": round(tuner.default_ms, 5),
+                          "seconds": round(time.time() - t0, 1), "line": line}), flush=True)
+        del tuner
+    return lines
+
+
+def _compile_quietly(tuner, tuning):
+    """Build (and cache) the plan of a candidate; failures are left to evaluate()."""
+    try:
+        make_plan(tuner.cfg, cache=tuner.cache, tuning=tuning)
+    except (BadConfiguration, BbfftError, ValueError):
+        pass
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("-o", "--output", default=None, help="table file to write (.inc)")
@@ -336,8 +434,19 @@ def main(argv=None):
     ap.add_argument("--confirm-reps", type=int, default=100, metavar="N",
                     help="launches per candidate and round of the confirmation; long runs rank "
                          "the candidates at the sustained clocks of a long job (0: --reps)")
-    ap.add_argument("descriptors", nargs="+")
+    ap.add_argument("--ragged-serving", action="store_true",
+                    help="tune the variants of the ragged serving workload instead of "
+                         "descriptors (table entries c2c-ragged)")
+    ap.add_argument("descriptors", nargs="*")
     a = ap.parse_args(argv)
+    if a.ragged_serving:
+        lines = tune_ragged_serving(reps=a.reps, confirm_rounds=a.confirm,
+                                    confirm_reps=a.confirm_reps or a.reps)
+        if a.output:
+            write_table(a.output, merge_table(a.output, lines) if a.merge else lines)
+        return 0
+    if not a.descriptors:
+        ap.error("descriptors are required (or --ragged-serving)")
     lines = []
     for desc in a.descriptors:
         t0 = time.time()

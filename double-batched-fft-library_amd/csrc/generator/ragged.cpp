@@ -7,9 +7,11 @@
 #include "bbfft/detail/generator_impl.hpp"
 #include "emitter.hpp"
 #include "heuristics.hpp"
+#include "tuning_table.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <iomanip>
 #include <map>
@@ -24,8 +26,9 @@ namespace {
 
 /// Work-items per work-group of the persistent kernel (at least this many).
 constexpr std::size_t ragged_min_group = 256;
-/// LDS budget of one variant (keeps two or more work-groups resident per CU).
-constexpr std::size_t ragged_lds_limit = 64 * 1024;
+/// LDS budget of one variant: 160 KB / 5, so that the five work-groups per CU that the
+/// occupancy target (5 waves per SIMD, 4 waves per group) allows also fit in LDS.
+constexpr std::size_t ragged_lds_limit = 32 * 1024;
 /// Largest radix of a variant. All variants share one kernel, whose register allocation is the
 /// largest of the variants', so one wide radix would lower the occupancy of every job.
 constexpr int ragged_radix_max = 12;
@@ -59,6 +62,7 @@ std::string ragged_kernel_configuration::identifier() const {
         all += v.identifier();
         all += ';';
     }
+    all += "w" + std::to_string(waves_per_eu);
     std::ostringstream oss;
     oss << "bbfft_ragged_f" << static_cast<int>(fp) * 8 << "_G" << group_size << "_V"
         << variants.size() << "_" << std::hex << std::setw(16) << std::setfill('0') << fnv1a(all);
@@ -107,15 +111,29 @@ ragged_kernel_configuration configure_ragged(ragged_configuration const &cfg,
         auto const &job = cfg.jobs[j];
         rk.job_variant[j] = index.at(key_t{job.N, job.M, static_cast<int>(job.dir)});
     }
-    // Factorisation, work-items per transform and M tile from the plain-kernel configuration
-    // (tuning table or heuristics), with the radix capped.
+    // Factorisation, work-items per transform, M tile, LDS padding and access hints: the
+    // table's ragged entry for (N, M) (tuned under this kernel's constraints), else the
+    // plain-kernel configuration (tuning table or heuristics) with the radix capped.
     std::vector<fft1d_tuning> tune(keys.size());
     std::size_t G = ragged_min_group;
     for (std::size_t v = 0; v < keys.size(); ++v) {
         auto const &[N, M, d] = keys[v];
+        fft1d_tuning &t = tune[v];
+        fft1d_tuning rt;
+        if (gen::lookup_tuning(cfg.fp, transform_type::c2c, N, M, gen::tuning_ragged, rt) &&
+            !rt.factorization.empty() && rt.Mb * rt.Nt <= ragged_min_group) {
+            t.factorization = rt.factorization;
+            t.Nt = rt.Nt;
+            t.Mb = rt.Mb;
+            t.lds_pad = rt.lds_pad;
+            t.no_pad = rt.lds_pad == 0;
+            t.flags = rt.flags;
+            t.has_flags = true;
+            G = std::max(G, t.Mb * t.Nt);
+            continue;
+        }
         auto b = configure_fft1d(variant_configuration(cfg.fp, M, N, static_cast<direction>(d)),
                                  info);
-        fft1d_tuning &t = tune[v];
         t.factorization = b.factorization;
         t.Nt = b.Nt;
         int const rmax = *std::max_element(b.factorization.begin(), b.factorization.end());
@@ -159,6 +177,11 @@ ragged_kernel_configuration configure_ragged(ragged_configuration const &cfg,
         rk.variants.push_back(c);
     }
     rk.group_size = G;
+    // occupancy target (BBFFT_RAGGED_WAVES overrides it for measurements)
+    rk.waves_per_eu = cfg.fp == precision::f32 ? 5u : 0u;
+    if (char const *w = std::getenv("BBFFT_RAGGED_WAVES")) {
+        rk.waves_per_eu = static_cast<unsigned>(std::atoi(w));
+    }
     // Table order: grouped by variant (work-groups running at the same time then tend to run
     // the same code), jobs of one variant in their given order.
     rk.order.resize(cfg.jobs.size());
@@ -252,7 +275,9 @@ void generate_ragged(std::ostream &os, ragged_kernel_configuration const &rk,
     // without spilling when asked to (measured on MI355X for the 4096-job serving mix: 78.4 ms
     // against 85.4 ms with radix <= 16 at 4 waves).
     std::string const occupancy =
-        rk.fp == precision::f32 ? " __attribute__((amdgpu_waves_per_eu(5)))" : "";
+        rk.waves_per_eu ? " __attribute__((amdgpu_waves_per_eu(" +
+                              std::to_string(rk.waves_per_eu) + ")))"
+                        : "";
     os << "extern \"C\" __global__ void __launch_bounds__(" << rk.group_size << ")"
        << occupancy << " " << n
        << "(const " << T2 << "* in, " << T2

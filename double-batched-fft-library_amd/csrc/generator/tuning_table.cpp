@@ -1,6 +1,7 @@
 // Persisted autotuning table: per (precision, transform type, N, M) the kernel parameters the
 // autotuner measured fastest on gfx950. The type may carry the input storage of half-input
-// kernels: c2c-f16, c2c-bf16, c2c-f16-mfma, c2c-bf16-mfma. Built into the library from tuning/gfx950.inc;
+// kernels: c2c-f16, c2c-bf16, c2c-f16-mfma, c2c-bf16-mfma; c2c-ragged entries hold the
+// variants of the persistent ragged kernel. Built into the library from tuning/gfx950.inc;
 // BBFFT_TUNING_FILE replaces it with a table file of the same format and BBFFT_TUNING=off
 // disables it (heuristics only).
 #include "tuning_table.hpp"
@@ -23,8 +24,9 @@ char const builtin_table[] =
 
 using key_t = std::tuple<int, int, std::size_t, std::size_t, unsigned>;
 
-/// Input-storage bits of a kernel variant that select a separate table entry.
-constexpr unsigned storage_bits = fft1d_input_f16 | fft1d_input_bf16 | fft1d_mfma;
+/// Input-storage bits of a kernel variant (and the ragged-variant bit) that select a separate
+/// table entry.
+constexpr unsigned storage_bits = fft1d_input_f16 | fft1d_input_bf16 | fft1d_mfma | tuning_ragged;
 
 std::map<key_t, fft1d_tuning> parse(std::string const &text) {
     std::map<key_t, fft1d_tuning> table;
@@ -51,7 +53,8 @@ std::map<key_t, fft1d_tuning> parse(std::string const &text) {
         unsigned storage = 0;
         for (auto [suffix, bit] : {std::pair<char const *, unsigned>{"-f16", fft1d_input_f16},
                                    {"-bf16", fft1d_input_bf16},
-                                   {"-mfma", fft1d_mfma}}) {
+                                   {"-mfma", fft1d_mfma},
+                                   {"-ragged", tuning_ragged}}) {
             auto pos = type.find(suffix);
             if (pos != std::string::npos) {
                 storage |= bit;

@@ -9,6 +9,11 @@
 
 namespace bbfft::gen {
 
+/// Key bit of the entries tuned for variants of the persistent ragged kernel (type
+/// "c2c-ragged" in the table): factorisation, work-items per transform, M tile, LDS padding
+/// and access hints measured under that kernel's constraints.
+constexpr unsigned tuning_ragged = 1u << 30;
+
 /// Tuned parameters for (precision, type, N, M) and the input storage bits of `variant`
 /// (half-precision input, MFMA stage), if the table has an entry. Only the access-hint flags
 /// are stored; the caller adds the variant bits.

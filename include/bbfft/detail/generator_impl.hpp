@@ -218,10 +218,11 @@ struct BBFFT_EXPORT ragged_kernel_configuration {
     std::vector<unsigned> job_variant;         ///< variant of each job
     std::vector<std::size_t> order;            ///< job index at each table position
     std::vector<std::uint64_t> tile_begin;     ///< first tile of each table position, + total
+    unsigned waves_per_eu = 0;                 ///< occupancy target of the kernel (0: none)
 
     std::uint64_t total_tiles() const { return tile_begin.empty() ? 0 : tile_begin.back(); }
-    /// Kernel symbol name; depends on the variants and the work-group size only, so the compiled
-    /// module serves any job list with the same variants.
+    /// Kernel symbol name; depends on the variants, the work-group size and the occupancy
+    /// target only, so the compiled module serves any job list with the same variants.
     std::string identifier() const;
     /// Bytes of LDS (the largest of the variants).
     std::size_t lds_bytes() const;
