@@ -48,6 +48,10 @@ def test_table_entries_are_valid_configurations():
         # access hints and the load-first tile loop; storage bits come from the type
         assert flags & ~(3 | bb.PREFETCH) == 0, ln
         assert not flags & bb.PREFETCH or typ in ("c2c", "r2c"), ln
+        if storage == ["ragged"]:  # variants of the persistent ragged kernel
+            assert base == "c2c" and prec == "f32" and not flags & bb.PREFETCH, ln
+            assert Mb * Nt <= autotune.RAGGED_GROUP, ln
+            continue
         if storage:  # half-input kernels: c2c fp32, MFMA entries start with radix 16
             assert base == "c2c" and prec == "f32" and set(storage) <= {"f16", "bf16", "mfma"}
             assert "mfma" not in storage or fac[0] == 16, ln
@@ -92,3 +96,29 @@ def test_benchmark_cli_k_selection():
     assert benchmark.k_for(A, 1, 1 << 28, 8, True) == 1
     A.k = 77
     assert benchmark.k_for(A, 8, 256, 4, True) == 77
+
+
+def test_ragged_entries_configure_the_serving_variants():
+    """configure_ragged takes the factorisation, work-items, M tile and padding of the table's
+    c2c-ragged entry for each (N, M) of the serving workload."""
+    from bbfft_amd import ragged
+    entries = {}
+    for ln in _table_lines():
+        prec, typ, N, M = autotune._key(ln)
+        if typ == "c2c-ragged":
+            f = ln.split("#")[0].split(":")[1].split()
+            entries[(int(N), int(M))] = ([int(v) for v in f[0].split("x")], int(f[1]),
+                                         int(f[2]), int(f[4]))
+    jobs = ragged.synthetic_serving_jobs()
+    var = ragged.job_variants(jobs)
+    seen = set()
+    for j, v in zip(jobs, var):
+        if (j.N, j.M) in seen:
+            continue
+        seen.add((j.N, j.M))
+        assert (j.N, j.M) in entries, (j.N, j.M)
+        fac, nt, mb, pad = entries[(j.N, j.M)]
+        vc = ragged.variant_configuration(jobs, v)
+        assert (vc["factorization"], vc["Nt"], vc["Mb"], vc["lds_pad"]) == (fac, nt, mb, pad)
+        assert vc["lds_bytes"] <= 32 * 1024
+    assert len(seen) == 32

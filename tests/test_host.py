@@ -277,6 +277,8 @@ def test_tuning_table_is_applied():
     entries = _table_entries()
     assert len(entries) >= 20
     for prec, typ, N, M, fac, Nt, Mb, Kb, pad, flags in entries:
+        if typ == "c2c-ragged":  # ragged variants: tests/test_autotune_host.py checks them
+            continue
         if "-" in typ:  # half-input kernels: c2c-{f16,bf16}[-mfma]
             _, storage, *mfma = typ.split("-")
             kc = bb.half_input_kernel_configuration(f"scfo{M}.{N}*65536", storage, bool(mfma))
