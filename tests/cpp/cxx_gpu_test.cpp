@@ -7,9 +7,11 @@
 #include "bbfft/hip/make_plan.hpp"
 #include "bbfft/jit_cache_all.hpp"
 #include "bbfft/parser.hpp"
+#include "bbfft/tensor_indexer.hpp"
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -34,7 +36,11 @@ static int c2c_case(char const *desc, bool inplace, bool events, hipStream_t str
                     jit_cache *cache) {
     configuration cfg = parse_fft_descriptor(desc);
     std::size_t const M = cfg.shape[0], N = cfg.shape[1], K = cfg.shape[2];
-    std::size_t const n_el = M * N * K;
+    // addressing of the (m, n, k) tensor through the public indexer (col-major: m fastest)
+    using indexer = tensor_indexer<std::size_t, 3, layout::col_major>;
+    indexer const xi({M, N, K}, {cfg.istride[0], cfg.istride[1], cfg.istride[2]});
+    indexer const yi({M, N, K}, {cfg.ostride[0], cfg.ostride[1], cfg.ostride[2]});
+    std::size_t const n_el = std::max(xi(M - 1, N - 1, K - 1), yi(M - 1, N - 1, K - 1)) + 1;
     std::vector<cplx> x(n_el), y(n_el);
     for (std::size_t k = 0; k < K; ++k) {
         for (std::size_t n = 0; n < N; ++n) {
@@ -42,7 +48,7 @@ static int c2c_case(char const *desc, bool inplace, bool events, hipStream_t str
                 std::size_t const b = (m + k) % N;
                 double const ph = tau * static_cast<double>((b * n) % N) / static_cast<double>(N);
                 double const s = (1.0 + static_cast<double>(k) / K) / static_cast<double>(N);
-                x[m * cfg.istride[0] + n * cfg.istride[1] + k * cfg.istride[2]] =
+                x[xi(m, n, k)] =
                     cplx(static_cast<float>(s * std::cos(ph)), static_cast<float>(s * std::sin(ph)));
             }
         }
@@ -78,7 +84,7 @@ static int c2c_case(char const *desc, bool inplace, bool events, hipStream_t str
             for (std::size_t m = 0; m < M; ++m) {
                 std::size_t const b = (m + k) % N;
                 double const ref = n == b ? 1.0 + static_cast<double>(k) / K : 0.0;
-                cplx const v = y[m * cfg.ostride[0] + n * cfg.ostride[1] + k * cfg.ostride[2]];
+                cplx const v = y[yi(m, n, k)];
                 err = std::max(err, std::abs(std::complex<double>(v.real() - ref, v.imag())));
             }
         }
@@ -149,6 +155,8 @@ int main() {
         failures += c2c_case("scfi8.256*512", true, false, stream, &cache);
         failures += c2c_case("scfo3.60*77", false, true, stream, &cache);
         failures += c2c_case("scfo8.256*512", false, true, stream, &cache); // cached kernel
+        // padded, non-packed strides in and out (M = 4 padded to 5, N to 18 rows)
+        failures += c2c_case("scfo4.16*8i1,5,90o1,6,108", false, false, stream, &cache);
         failures += r2c_case("srfo128*300", stream, &cache);
         failures += r2c_case("srfo27*33", stream, &cache);
     } catch (std::exception const &e) {

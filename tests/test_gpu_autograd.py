@@ -1,7 +1,6 @@
 """Differentiation on the GPU: the adjoint kernels behind torch.autograd (bbfft_amd.autograd)
 checked against FP64 finite differences (torch.autograd.gradcheck, and central differences on
 the 3-D filter pipeline) and against torch.fft's gradients."""
-import time
 
 import numpy as np
 import pytest
@@ -135,26 +134,27 @@ def test_3d_filter_pipeline_gradient():
 
 
 def test_3d_backward_forward_time_ratio():
+    """The hand-written adjoint keeps the backward pass within the 1.2x-of-forward target on
+    the 32 x 128^3 filter pipeline (median of 20 event-timed repetitions each)."""
     n, B = 128, 32
     x = torch.randn(B, n, n, n, device="cuda", dtype=torch.float32, requires_grad=True)
     f = _filter(n)
-    for _ in range(2):
+    for _ in range(3):
         loss, _ = _pipeline(x, f)
         torch.autograd.grad(loss, x)
     torch.cuda.synchronize()
-    tf = tb = 0.0
-    reps = 5
-    for _ in range(reps):
-        t0 = time.perf_counter()
+    tf, tb = [], []
+    for _ in range(20):
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
         loss, _ = _pipeline(x, f)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
+        e1.record()
         torch.autograd.grad(loss, x)
+        e2.record()
         torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        tf += t1 - t0
-        tb += t2 - t1
-    ratio = tb / tf
-    print(f"3-D filter pipeline: forward {1e3 * tf / reps:.3f} ms, backward "
-          f"{1e3 * tb / reps:.3f} ms, ratio {ratio:.3f}")
-    assert ratio < 2.0
+        tf.append(e0.elapsed_time(e1))
+        tb.append(e1.elapsed_time(e2))
+    mf, mb = float(np.median(tf)), float(np.median(tb))
+    ratio = mb / mf
+    print(f"3-D filter pipeline: forward {mf:.3f} ms, backward {mb:.3f} ms, ratio {ratio:.3f}")
+    assert ratio <= 1.2

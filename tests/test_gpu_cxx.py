@@ -1,7 +1,8 @@
 """The C++ drop-in API on the GPU: tests/cpp/cxx_gpu_test.cpp (built by
 __graft_entry__.build() against the in-tree library) plans and executes c2c and r2c transforms
 through bbfft::make_plan / plan::execute, out of place, in place and with caller-owned HIP
-events, and checks the reference tests' known answers."""
+events, addressing its tensors (padded strides included) with bbfft::tensor_indexer, and checks
+the reference tests' known answers."""
 import os
 import subprocess
 
@@ -22,4 +23,4 @@ def test_cxx_api_on_gpu():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith("PASSED")
-    assert r.stdout.count(" ok") == 6
+    assert r.stdout.count(" ok") == 7
