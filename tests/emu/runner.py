@@ -159,3 +159,86 @@ def pass_type(cfg, pass_index):
     if t == 1:
         return 1 if pass_index == 0 else 0
     return 2 if pass_index == cfg.dim - 1 else 0
+
+
+def race_check(source, call, in_bytes, out_bytes, grid, block, in_place=False, table=None,
+               clang=False):
+    """Run a generated kernel once under ThreadSanitizer (see hip_emu.hpp): `call` is the
+    kernel call with `in`, `out` (and `table`) in scope, e.g. "k((const float2*)in,
+    (float2*)out, 4ul)". Buffers are zero-filled (the kernels' access patterns do not depend on
+    the data) and end at an inaccessible page, so accesses past them are reported too; `table`
+    (uint64 array) is the ragged job table. Returns (ok, report): ok is False when
+    ThreadSanitizer reported a race, an access left the buffers, or the run failed."""
+    import hashlib as _h
+    os.makedirs(_BUILD, exist_ok=True)
+    tab = ""
+    if table is not None:
+        vals = ",".join(f"{int(v)}ul" for v in table)
+        tab = f"static const unsigned long table_data[] = {{{vals}}};\n"
+    # Each buffer ends where an inaccessible guard page begins, so an access past its end
+    # faults (reported as a bounds error) instead of reading or writing whatever follows.
+    tu = ('#include "hip_emu.hpp"\n#include <csignal>\n#include <cstdlib>\n#include <cstring>\n'
+          '#include <sys/mman.h>\n#include <unistd.h>\n' + source + "\n" + tab +
+          "static void* guarded(std::size_t bytes) {\n"
+          "  std::size_t const pg = 4096, span = (bytes + pg - 1) / pg * pg;\n"
+          "  char* p = (char*)mmap(nullptr, span + pg, PROT_READ | PROT_WRITE,\n"
+          "                        MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);\n"
+          "  mprotect(p + span, pg, PROT_NONE);\n"
+          "  return p + span - bytes;\n}\n"
+          "static void on_fault(int) {\n"
+          '  const char m[] = "bounds error: access outside the kernel\'s buffers\\n";\n'
+          "  write(2, m, sizeof(m) - 1);\n  _exit(67);\n}\n"
+          "int main() {\n"
+          "  std::signal(SIGSEGV, on_fault);\n"
+          f"  void* in = guarded({max(in_bytes, 16)});\n" +
+          ("  void* out = in;\n" if in_place else f"  void* out = guarded({max(out_bytes, 16)});\n") +
+          ("  const unsigned long* table = table_data;\n" if table is not None else "") +
+          f"  int rc = emu::launch({grid}u, {block}u, [&] {{ {call}; }});\n"
+          '  std::printf("launch rc %d\\n", rc);\n'
+          "  std::fflush(stdout);\n"
+          "  _exit(rc); // races are reported as they happen; skip the sanitizer's exit path\n"
+          "}\n")
+    with open(os.path.join(_HERE, "hip_emu.hpp"), "rb") as f:
+        shim = f.read()
+    cxx = CLANG if clang or "BBFFT_HIP_HALF_PRELUDE" in tu else "g++"
+    h = _h.sha1(tu.encode() + shim + cxx.encode()).hexdigest()[:16]
+    exe = os.path.join(_BUILD, f"race_{h}")
+    if not os.path.exists(exe):
+        src = os.path.join(_BUILD, f"race_{h}.{os.getpid()}.cpp")
+        with open(src, "w") as f:
+            f.write(tu)
+        tmp = exe + f".tmp{os.getpid()}"
+        subprocess.run([cxx, "-O1", "-g", "-std=c++17", "-ffp-contract=off", "-w",
+                        "-fsanitize=thread", "-I", _HERE, src, "-o", tmp], check=True)
+        os.replace(tmp, exe)
+        os.remove(src)
+    env = dict(os.environ, TSAN_OPTIONS=os.environ.get(
+        "BBFFT_RACE_TSAN_OPTIONS", "halt_on_error=1 exitcode=66 report_signal_unsafe=0 handle_segv=0"))
+    r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=600)
+    report = r.stdout + r.stderr + f"\n{exe}: exit status {r.returncode}\n"
+    ok = r.returncode == 0 and "ThreadSanitizer" not in report
+    return ok, report
+
+
+_tsan = None
+
+
+def have_tsan():
+    """Whether the host compiler builds and runs ThreadSanitizer programs here."""
+    global _tsan
+    if _tsan is None:
+        os.makedirs(_BUILD, exist_ok=True)
+        src = os.path.join(_BUILD, f"tsan_probe.{os.getpid()}.cpp")
+        exe = src[:-4]
+        with open(src, "w") as f:
+            f.write("int main() { return 0; }\n")
+        try:
+            subprocess.run(["g++", "-fsanitize=thread", src, "-o", exe], check=True,
+                           capture_output=True)
+            _tsan = subprocess.run([exe], capture_output=True).returncode == 0
+        except (OSError, subprocess.CalledProcessError):
+            _tsan = False
+        for p in (src, exe):
+            if os.path.exists(p):
+                os.remove(p)
+    return _tsan
