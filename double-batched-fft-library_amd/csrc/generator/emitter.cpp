@@ -322,7 +322,8 @@ class kernel_gen {
                bool alias_in = false, bool alias_out = false, body_options const &opt = {})
         : c_(c), f_{c.fp}, name_(std::move(name)), group_size_(group_size), alias_in_(alias_in),
           alias_out_(alias_out), lds_tables_(opt.lds_tables), lds_barriers_(opt.lds_barriers),
-          extra_param_(opt.extra_param), after_input_(opt.after_input) {
+          extra_param_(opt.extra_param), after_input_(opt.after_input),
+          input_index_xor_(opt.input_index_xor) {
         nfft_ = c.N_fft();
         L_ = c.factorization.size();
         real_ = c.type != transform_type::c2c;
@@ -925,7 +926,13 @@ class kernel_gen {
     }
 
     /// Load z[n] of the complex FFT input from global memory into variable v.
-    void emit_global_input(std::string const &v, std::string const &n) {
+    void emit_global_input(std::string const &v, std::string const &n0) {
+        std::string const n =
+            input_index_xor_.empty() ? n0 : "((" + n0 + ") ^ " + input_index_xor_ + ")";
+        if (!input_index_xor_.empty() && !(c_.type == transform_type::c2c && !half_in_) &&
+            !(half_ && vector_real_in())) {
+            throw bad_configuration("permuted input rows need complex element reads");
+        }
         if (c_.type == transform_type::c2c) {
             if (half_in_) {
                 w_.line(v + " = " + std::string(bf16_ ? "bf16c_to_f2(" : "f16c_to_f2(") +
@@ -1343,6 +1350,9 @@ class kernel_gen {
     /// c2r: Hermitian input -> complex FFT input z in LDS (reference: sbfft_gen.cpp:221-247
     /// for even N, 318-351 for odd N; the imaginary part of the DC bin is ignored).
     void emit_c2r_pre() {
+        if (!input_index_xor_.empty()) {
+            throw bad_configuration("permuted input rows are not supported for c2r");
+        }
         w_.line("// c2r pre-pass");
         std::size_t const H = nfft_;
         std::size_t const Q = half_ ? H / 2 + 1 : (c_.N - 1) / 2 + 1;
@@ -1497,6 +1507,7 @@ class kernel_gen {
     bool lds_barriers_ = false; // barriers order LDS only (fused kernels)
     std::string extra_param_;  // extra parameter of a tile body
     std::string after_input_;  // code run once stage 0 has read all of the tile's input
+    std::string input_index_xor_; // XOR-ed into input element indices (permuted input rows)
     int direct_ = 0; // radix of the direct DFT stage (0: none)
     bool local32_ = false;
     bool prefetch_ = false;

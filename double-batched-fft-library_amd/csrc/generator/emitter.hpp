@@ -27,6 +27,9 @@ struct body_options {
     std::string extra_param;
     /// Statement emitted once stage 0 has read all of the tile's input (after its barrier).
     std::string after_input;
+    /// Expression XOR-ed into the element index of every input read (c2c, or r2c with
+    /// even N read as complex pairs): the input sits in a row-wise permuted layout.
+    std::string input_index_xor;
 };
 /// Tile body whose input (alias_in) or output (alias_out) pointer may point into the LDS
 /// scratch passed to the body (real transforms included).

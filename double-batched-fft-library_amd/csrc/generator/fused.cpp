@@ -130,7 +130,8 @@ std::string fused2_configuration::identifier() const {
                    ";" + std::to_string(stage_words) + ";" + std::to_string(stage_load_bytes) +
                    ";" + std::to_string(lds_barriers) + ";" + std::to_string(lds_tables) +
                    ";" + std::to_string(waves_per_eu) + ";" + std::to_string(copy_wave0) + ";" +
-                   std::to_string(early_copy));
+                   std::to_string(early_copy) + ";" + std::to_string(stage_swizzle_rows) + ";" +
+                   std::to_string(stage_swizzle_elems));
     return oss.str();
 }
 
@@ -168,7 +169,11 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
     fused2_configuration fc = {};
     // As many work-items as the device allows, so each holds few elements (smallest radices);
     // streaming hints on the global side only.
-    std::size_t const wg_max = std::min<std::size_t>(1024, info.max_work_group_size);
+    std::size_t wg_max = std::min<std::size_t>(1024, info.max_work_group_size);
+    if (!env("BBFFT_FUSE_WGMAX").empty()) { // A/B: smaller groups (several per CU)
+        wg_max = std::min<std::size_t>(
+            wg_max, static_cast<std::size_t>(std::atoi(env("BBFFT_FUSE_WGMAX").c_str())));
+    }
     if (!one_tile(pa, ka, info, fft1d_nontemporal_load, variant_a, wg_max, fc.a) ||
         !one_tile(pb, kb, info, fft1d_nontemporal_store, variant_b, wg_max, fc.b)) {
         return false;
@@ -247,6 +252,26 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
             // only useful when barriers do not wait for it
             st.early_copy = option("BBFFT_FUSE_EARLY", "on", r2c_first) &&
                             (st.a.factorization.size() > 1 || st.a.type == transform_type::r2c);
+            // Rows of pass a read from the stage: a wave holds 64 / Nt rows and reads one
+            // segment of Nt complex elements of each per instruction. Rows of a multiple of
+            // (64 / Nt) segments alias in the LDS banks; permuting the segments row by row
+            // spreads them. Measured (bench.py's 3-D legs, processes alternated): 32 x 128^3
+            // r2c 0.2107 ms against 0.2166 with the plain layout; a c2c first pass
+            // (32 x 128 x 128 x 64) 0.2266 against 0.2220, so r2c first passes only
+            // (BBFFT_FUSE_SWIZZLE=on|off overrides).
+            std::size_t const Nt = st.a.Nt;
+            std::size_t const eb = 2 * static_cast<std::size_t>(pa.fp); // complex element
+            bool const complex_rows =
+                pa.istride[1] == 1 && !pa.callbacks &&
+                (pa.type == transform_type::c2c ||
+                 (pa.type == transform_type::r2c && Na % 2 == 0 && pa.istride[2] % 2 == 0));
+            std::size_t const row_bytes = pa.istride[2] * in_bytes;
+            if (option("BBFFT_FUSE_SWIZZLE", "on", r2c_first) && complex_rows && Ma == 1 &&
+                st.a.Mb == 1 && Nt >= 2 && Nt <= 32 && 64 % Nt == 0 && (Nt * eb) % 16 == 0 &&
+                row_bytes % ((64 / Nt) * Nt * eb) == 0 && W % 4 == 0) {
+                st.stage_swizzle_rows = static_cast<unsigned>(64 / Nt);
+                st.stage_swizzle_elems = static_cast<unsigned>(Nt);
+            }
             fc = st;
         }
     }
@@ -311,6 +336,20 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
         std::size_t const wv0 = fc.copy_wave0; // copying waves: wv0 .. wv0 + NWc - 1
         std::size_t const NWc = fc.group_size >= 64 ? fc.group_size / 64 - wv0 : 1;
         std::size_t const NB = (W + BS - 1) / BS;
+        // Source word of stage word i: i itself, or with stage_swizzle_rows the word of the
+        // same row whose 16-byte chunk index is XOR-ed with (row % R) * (segment chunks).
+        std::string src_word = "i";
+        if (fc.stage_swizzle_rows) {
+            std::size_t const in_b = fc.a.type == transform_type::r2c
+                                         ? static_cast<std::size_t>(fc.a.fp)
+                                         : 2 * static_cast<std::size_t>(fc.a.fp);
+            std::size_t const CR = fc.a.istride[2] * in_b / 16; // 16-byte chunks per row
+            std::size_t const SC = fc.stage_swizzle_elems * 2 * static_cast<std::size_t>(fc.a.fp) / 16;
+            src_word = "((i / " + std::to_string(4 * CR) + "u) * " + std::to_string(4 * CR) +
+                       "u + ((((i / 4u) % " + std::to_string(CR) + "u) ^ (((i / " +
+                       std::to_string(4 * CR) + "u) % " + std::to_string(fc.stage_swizzle_rows) +
+                       "u) * " + std::to_string(SC) + "u)) * 4u) + i % 4u)";
+        }
         os << "// copy of one plane's input (src, or nothing when src is null) into the staging\n"
            << "// buffer `stage` (LDS)\n"
            << "static __device__ __forceinline__ void " << copy
@@ -318,6 +357,10 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
            << "  const unsigned ln = threadIdx.x % 64u;\n"
            << "  const unsigned wv = threadIdx.x / 64u - " << wv0 << "u; // copying wave index\n"
            << "  if (!src || wv >= " << NWc << "u) return; // other waves copy nothing\n";
+        if (fc.stage_swizzle_rows) {
+            os << "  // rows permuted in 16-byte chunks: " << fc.stage_swizzle_rows
+               << " rows read together land in different LDS banks\n";
+        }
         if (fc.stage_load_bytes == 16) {
             // 16 bytes per work-item: wave block b of 256 words lands at stage + 256 b, lane
             // l's four words at + 4 l (the pointer's alignment is uniform: one branch)
@@ -325,14 +368,16 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
             os << "  if ((reinterpret_cast<unsigned long>(src) & 15ul) == 0ul) {\n"
                << "    for (unsigned b = wv; b < " << NB16 << "u; b += " << NWc << "u) {\n"
                << "      const unsigned i = b * 256u + 4u * ln;\n"
-               << "      if (i < " << W << "u) BBFFT_LDS_DMA16(src + i, stage + b * 256u);\n"
+               << "      if (i < " << W << "u) BBFFT_LDS_DMA16(src + " << src_word
+               << ", stage + b * 256u);\n"
                << "    }\n"
                << "    return;\n"
                << "  }\n";
         }
         os << "  for (unsigned b = wv; b < " << NB << "u; b += " << NWc << "u) {\n"
            << "    const unsigned i = b * " << BS << "u + ln;\n"
-           << "    if (i < " << W << "u) BBFFT_LDS_DMA4(src + i, stage + b * " << BS << "u);\n"
+           << "    if (i < " << W << "u) BBFFT_LDS_DMA4(src + " << src_word << ", stage + b * "
+           << BS << "u);\n"
            << "  }\n"
            << "}\n";
     }
@@ -343,6 +388,10 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
     if (fc.stage_words && fc.early_copy) {
         oa.extra_param = "const float* next_input, float* stage";
         oa.after_input = copy + "(next_input, stage);";
+    }
+    if (fc.stage_words && fc.stage_swizzle_rows) {
+        oa.input_index_xor = "(" + std::to_string(fc.stage_swizzle_elems) + "u * (kl % " +
+                             std::to_string(fc.stage_swizzle_rows) + "u))";
     }
     generate_fft1d_body_aliased(os, fc.a, fa, fc.group_size, false, true, oa);
     generate_fft1d_body_aliased(os, fc.b, fb, fc.group_size, true, false, ob);

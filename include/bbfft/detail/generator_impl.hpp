@@ -185,6 +185,14 @@ struct BBFFT_EXPORT fused2_configuration {
     /// The next plane's staging loads start as soon as pass a's first stage has read the
     /// staged input (instead of after pass a).
     bool early_copy = false;
+    /// Staged input rows whose length is a multiple of 512 bytes alias in the LDS banks when a
+    /// wave reads one segment of each of several rows at once. With stage_swizzle_rows = R
+    /// (a power of two, 0 = off), the 16-byte chunks of staged row r are permuted: segment s
+    /// (stage_swizzle_elems complex elements) of the row holds the row's segment
+    /// s ^ (r % R), so the rows read together land in different banks. Pass a applies the same
+    /// XOR to its input element indices.
+    unsigned stage_swizzle_rows = 0;
+    unsigned stage_swizzle_elems = 0;
 
     std::string identifier() const;
     std::size_t lds_bytes() const; ///< plane and stage scratch (without the staging buffer)

@@ -95,7 +95,7 @@ def test_fused_staged_matches_unstaged(t, M, dims, K, monkeypatch):
 
 # every variant setting off; VARIANTS override it (an empty dict: the plan's defaults)
 PLAIN = {"BBFFT_FUSE_DMA": "4", "BBFFT_FUSE_SYNC": "full", "BBFFT_FUSE_TABLES": "const",
-         "BBFFT_FUSE_EARLY": "off", "BBFFT_FUSE_DMAWAVES": "all"}
+         "BBFFT_FUSE_EARLY": "off", "BBFFT_FUSE_DMAWAVES": "all", "BBFFT_FUSE_SWIZZLE": "off"}
 VARIANTS = [
     None,
     {"BBFFT_FUSE_DMA": "16", "BBFFT_FUSE_SYNC": "lds", "BBFFT_FUSE_TABLES": "lds"},
@@ -155,6 +155,40 @@ def test_fused_variants_match_default(t, M, dims, K, variant, monkeypatch):
         assert "(next_input, stage);" in src
     y1 = _run_plan(cfg, x, n_out)
     assert np.array_equal(y0, y1)
+
+
+@pytest.mark.parametrize("t,M,dims,K", [(1, 1, [128, 16], 3), (0, 1, [64, 16], 3),
+                                        (1, 1, [256, 8], 2)])
+def test_fused_swizzled_stage_matches_plain(t, M, dims, K, monkeypatch):
+    """Rows of 512 bytes (or a multiple) sit in the staging buffer with their 64-byte segments
+    permuted row by row (the 8 rows a wave reads at once then use different LDS banks); the
+    kernel gives the same bits as with the plain layout and as the unstaged kernel."""
+    cfg = _cfg(M, dims, K, 4, t)
+    shape = cfg.shape
+    D = len(dims)
+    half = list(shape)
+    half[1] = dims[0] // 2 + 1
+    oshape = half if t == 1 else shape
+    n_in = oracle.extent(D, shape, cfg.istride)
+    n_out = oracle.extent(D, oshape, cfg.ostride)
+    x = oracle.uniform_real(6, n_in) if t == 1 else oracle.uniform_complex(6, n_in)
+    x = x.astype(np.float32 if t == 1 else np.complex64)
+    ys = []
+    for env in ({"BBFFT_FUSE_SWIZZLE": "on"}, {"BBFFT_FUSE_SWIZZLE": "off"},
+                {"BBFFT_FUSE_STAGE": "off"}, {}):
+        for k in ("BBFFT_FUSE_SWIZZLE", "BBFFT_FUSE_STAGE"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        src = bb.generate_fused_source(cfg)
+        # on by default for r2c first passes
+        swz = env.get("BBFFT_FUSE_SWIZZLE") == "on" or (not env and t == 1)
+        assert ("rows permuted" in src) == swz
+        ys.append(_run_plan(cfg, x, n_out))
+    assert all(np.array_equal(ys[0], y) for y in ys[1:])
+    ref = oracle.fft(D, shape, cfg.istride, cfg.ostride, t, int(cfg.dir),
+                     x.astype(np.float64 if t == 1 else np.complex128), out_size=n_out)
+    assert oracle.rel_l2(ys[0], ref) <= oracle.tolerance(4, int(np.prod(dims)))
 
 
 def test_fused_in_place_layout_r2c():

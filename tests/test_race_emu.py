@@ -94,14 +94,15 @@ def _fused(M, dims, K, t, env=None):
 FUSED_VARIANTS = {
     "default": {},
     "plain": {"BBFFT_FUSE_DMA": "4", "BBFFT_FUSE_SYNC": "full", "BBFFT_FUSE_TABLES": "const",
-              "BBFFT_FUSE_EARLY": "off", "BBFFT_FUSE_DMAWAVES": "all"},
+              "BBFFT_FUSE_EARLY": "off", "BBFFT_FUSE_DMAWAVES": "all", "BBFFT_FUSE_SWIZZLE": "off"},
     "unstaged": {"BBFFT_FUSE_STAGE": "off"},
 }
 
 
 @pytest.mark.parametrize("variant", sorted(FUSED_VARIANTS))
 @pytest.mark.parametrize("t,M,dims,K", [(1, 1, [16, 64], 9), (2, 1, [32, 16, 4], 3),
-                                        (0, 1, [8, 32, 3], 5), (1, 1, [32, 24], 5)])
+                                        (0, 1, [8, 32, 3], 5), (1, 1, [32, 24], 5),
+                                        (1, 1, [128, 16], 3)])
 def test_fused_kernels_are_race_free(t, M, dims, K, variant, monkeypatch):
     """Fused plane kernels, several planes per work-group: the staged input of the next plane
     lands in LDS while the current plane's second pass runs; the plane shares the LDS with
