@@ -115,13 +115,22 @@ ragged_kernel_configuration configure_ragged(ragged_configuration const &cfg,
     // table's ragged entry for (N, M) (tuned under this kernel's constraints), else the
     // plain-kernel configuration (tuning table or heuristics) with the radix capped.
     std::vector<fft1d_tuning> tune(keys.size());
+    // BBFFT_RAGGED_GROUP / BBFFT_RAGGED_LDS override the common group size and the LDS budget
+    // of a variant (measurements)
     std::size_t G = ragged_min_group;
+    std::size_t lds_limit = ragged_lds_limit;
+    if (char const *g = std::getenv("BBFFT_RAGGED_GROUP")) {
+        G = std::max<std::size_t>(64, static_cast<std::size_t>(std::atoi(g)));
+    }
+    if (char const *l = std::getenv("BBFFT_RAGGED_LDS")) {
+        lds_limit = static_cast<std::size_t>(std::atoi(l));
+    }
     for (std::size_t v = 0; v < keys.size(); ++v) {
         auto const &[N, M, d] = keys[v];
         fft1d_tuning &t = tune[v];
         fft1d_tuning rt;
         if (gen::lookup_tuning(cfg.fp, transform_type::c2c, N, M, gen::tuning_ragged, rt) &&
-            !rt.factorization.empty() && rt.Mb * rt.Nt <= ragged_min_group) {
+            !rt.factorization.empty() && rt.Mb * rt.Nt <= std::max(G, ragged_min_group)) {
             t.factorization = rt.factorization;
             t.Nt = rt.Nt;
             t.Mb = rt.Mb;
@@ -160,7 +169,7 @@ ragged_kernel_configuration configure_ragged(ragged_configuration const &cfg,
         for (;;) {
             c = configure_fft1d(variant_configuration(cfg.fp, M, N, static_cast<direction>(d)),
                                 info, t);
-            if (c.lds_bytes() <= ragged_lds_limit || t.Kb == 1) {
+            if (c.lds_bytes() <= lds_limit || t.Kb == 1) {
                 break;
             }
             t.Kb = t.Kb / 2;
