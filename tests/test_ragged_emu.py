@@ -85,6 +85,22 @@ def test_ragged_matches_plain_kernels_bitwise():
         assert np.array_equal(plain, y[j.out_offset:j.out_offset + j.elements]), j
 
 
+@pytest.mark.parametrize("group,lds", [(512, 81920), (1024, 163840)])
+def test_larger_common_group_same_bits(group, lds, monkeypatch):
+    """A larger common work-group (K tiles widened to fill it) computes the same bits as the
+    default 256-work-item kernel."""
+    jobs = _mixed_jobs(6, 8, (12, 60, 243, 500, 512), kmax=6)
+    n = ragged.total_elements(jobs)
+    x = oracle.uniform_complex(6, n).astype(np.complex64)
+    y0 = runner.run_ragged(ragged, jobs, x, n, grid=3)
+    monkeypatch.setenv("BBFFT_RAGGED_GROUP", str(group))
+    monkeypatch.setenv("BBFFT_RAGGED_LDS", str(lds))
+    kc = ragged.kernel_configuration(jobs, 4)
+    assert kc["group_size"] == group
+    y1 = runner.run_ragged(ragged, jobs, x, n, grid=3)
+    assert np.array_equal(y0, y1)
+
+
 def test_job_table_layout():
     jobs = ragged.pack([ragged.Job(2, 60, 100, -1), ragged.Job(1, 12, 7, 1),
                         ragged.Job(2, 60, 3, -1), ragged.Job(3, 128, 50, 1)])
