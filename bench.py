@@ -322,7 +322,10 @@ def autograd_3d_filter(torch, steps):
     """Differentiated 3-D pipeline on 32 x 128^3 fp32 (normal inputs): r2c, multiply by a
     seeded complex 128 x 128 x 65 filter, c2r / 128^3, loss = sum of squares; forward and
     backward (hand-written adjoint kernels through torch.autograd) timed separately with HIP
-    events on the current stream."""
+    events on the current stream. The product with the filter and the 1/128^3 scale run inside
+    the c2r's first pass (load callback) and, in the backward, the adjoint's last pass (store
+    callback); the same pipeline with the product and scale as separate torch operations is
+    timed beside it ("unfused")."""
     from bbfft_amd import autograd as ag
     n, B = 128, 32
     g = torch.Generator(device="cuda")
@@ -331,29 +334,40 @@ def autograd_3d_filter(torch, steps):
     f = torch.complex(torch.randn(n, n, n // 2 + 1, generator=g, device="cuda"),
                       torch.randn(n, n, n // 2 + 1, generator=g, device="cuda"))
 
-    def forward():
+    def fused():
+        z = ag.irfftn(ag.rfftn(x, 3), n, 3, filter=f, scale=1.0 / n ** 3)
+        return (z * z).sum()
+
+    def unfused():
         z = ag.irfftn(ag.rfftn(x, 3) * f, n, 3) / n ** 3
         return (z * z).sum()
 
-    for _ in range(2):
-        torch.autograd.grad(forward(), x)
-    torch.cuda.synchronize()
-    reps = max(3, min(20, steps // 10))
-    tf = tb = 0.0
-    for _ in range(reps):
-        s, m, e = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-        s.record()
-        loss = forward()
-        m.record()
-        torch.autograd.grad(loss, x)
-        e.record()
+    def timed(forward, reps):
+        for _ in range(2):
+            torch.autograd.grad(forward(), x)
         torch.cuda.synchronize()
-        tf += s.elapsed_time(m)
-        tb += m.elapsed_time(e)
+        tf = tb = 0.0
+        for _ in range(reps):
+            s, m, e = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            s.record()
+            loss = forward()
+            m.record()
+            torch.autograd.grad(loss, x)
+            e.record()
+            torch.cuda.synchronize()
+            tf += s.elapsed_time(m)
+            tb += m.elapsed_time(e)
+        return tf / reps, tb / reps
+
+    reps = max(3, min(20, steps // 10))
+    tf, tb = timed(fused, reps)
+    uf, ub = timed(unfused, reps)
     del x, f
     torch.cuda.empty_cache()
-    return {"forward_ms": round(tf / reps, 4), "backward_ms": round(tb / reps, 4),
-            "backward_over_forward": round(tb / tf, 3), "reps": reps}
+    return {"forward_ms": round(tf, 4), "backward_ms": round(tb, 4),
+            "backward_over_forward": round(tb / tf, 3), "reps": reps,
+            "unfused": {"forward_ms": round(uf, 4), "backward_ms": round(ub, 4),
+                        "backward_over_forward": round(ub / uf, 3)}}
 
 
 def cpu_baseline(budget_s=15.0):

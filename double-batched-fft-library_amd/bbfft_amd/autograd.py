@@ -2,6 +2,7 @@
 
     X = autograd.rfftn(x, ndim=3)          # real (..., N3, N2, N1) -> complex (..., N3, N2, N1/2+1)
     x = autograd.irfftn(X, n1=128, ndim=3) # complex half spectrum -> real, unnormalised
+    y = autograd.irfftn(X, 128, 3, filter=H, scale=128 ** -3)  # irfftn(X * H) * scale, fused
     Y = autograd.fftn(x, ndim=1, direction=-1)
 
 Tensors are torch tensors in row-major order whose trailing `ndim` axes are the FFT modes (the
@@ -17,8 +18,8 @@ import math
 
 import torch
 
-from . import configuration, default_cache, direction, make_adjoint_plan, make_plan, precision
-from . import transform_type
+from . import configuration, default_cache, direction, kernel_language, make_adjoint_plan
+from . import make_plan, precision, transform_type, user_module
 
 _plans = {}
 
@@ -108,9 +109,15 @@ def rfftn(x, ndim=1):
     return _Transform.apply(x, cfg, out_shape, _complex_of(fp), x.shape, x.dtype)
 
 
-def irfftn(X, n1, ndim=1):
+def irfftn(X, n1, ndim=1, filter=None, scale=None):
     """Unnormalised backward complex-to-real transform; `n1` is the real length of the last
-    axis (X's last axis must be n1 // 2 + 1)."""
+    axis (X's last axis must be n1 // 2 + 1).
+
+    With `filter` (a complex tensor of X's trailing `ndim` axes, applied to every batch) and/or
+    `scale` (a number) it computes irfftn(X * filter) * scale, the product taken by the
+    transform's first pass as it loads X (a load callback) instead of a separate pass over the
+    spectrum; its backward multiplies by conj(filter) * scale as the adjoint's last pass stores
+    (a store callback). The filter is a constant: it gets no gradient."""
     if not X.is_complex():
         raise TypeError("irfftn expects a complex tensor")
     if X.shape[-1] != n1 // 2 + 1:
@@ -120,4 +127,94 @@ def irfftn(X, n1, ndim=1):
     fft_rev[-1] = n1
     out_shape = list(X.shape[:-1]) + [n1]
     cfg = _cfg("c2r", fft_rev[::-1], batch, fp, direction.backward)
-    return _Transform.apply(X, cfg, out_shape, _real_of(fp), X.shape, X.dtype)
+    if filter is None and scale is None:
+        return _Transform.apply(X, cfg, out_shape, _real_of(fp), X.shape, X.dtype)
+    if filter is not None:
+        if filter.dtype != X.dtype or filter.device != X.device:
+            raise TypeError("the filter must have X's dtype and device")
+        if tuple(filter.shape) != tuple(X.shape[X.dim() - ndim:]):
+            raise ValueError("the filter must have the shape of X's last ndim axes")
+        if filter.requires_grad:
+            raise ValueError("the filter is a constant (it gets no gradient)")
+        filter = filter.contiguous()
+    return _FilteredC2R.apply(X, filter, 1.0 if scale is None else float(scale), cfg,
+                              out_shape)
+
+
+# Callback source of the fused spectral product. The filter's device address and the scale are
+# literals of the source (one kernel per filter buffer and scale, cached like any kernel);
+# offsets index the spectrum, whose trailing modes repeat every `period` elements. The
+# remainder is taken in 32-bit arithmetic when the spectrum has fewer than 2^32 elements (a
+# 64-bit remainder per element costs the 16-element work-items enough registers to spill).
+_FILTER_SOURCE = """
+__device__ {t2} bbfft_filter_load({t2}* in, size_t offset) {{
+    const {t2}* h = (const {t2}*){addr:#x}ul;
+    const {t2} a = in[offset];
+    {t2} w;
+    w.x = 1;
+    w.y = 0;
+    if ({has_filter}) w = h[{index}];
+    {t2} r;
+    r.x = (a.x * w.x - a.y * w.y) * ({t}){scale};
+    r.y = (a.x * w.y + a.y * w.x) * ({t}){scale};
+    return r;
+}}
+__device__ void bbfft_filter_store({t2}* out, size_t offset, {t2} v) {{
+    const {t2}* h = (const {t2}*){addr:#x}ul;
+    {t2} w;
+    w.x = 1;
+    w.y = 0;
+    if ({has_filter}) w = h[{index}];
+    {t2} r;
+    r.x = (v.x * w.x + v.y * w.y) * ({t}){scale};
+    r.y = (v.y * w.x - v.x * w.y) * ({t}){scale};
+    out[offset] = r;
+}}
+"""
+
+
+def _filter_plan(cfg, filt, scale, adjoint):
+    """c2r plan loading X * filt * scale, or its adjoint storing (.) * conj(filt) * scale."""
+    stream = torch.cuda.current_stream().cuda_stream
+    addr = filt.data_ptr() if filt is not None else 0
+    period = filt.numel() if filt is not None else 1
+    key = (cfg.to_string(), adjoint, stream, "filter", addr, period, scale)
+    p = _plans.get(key)
+    if p is None:
+        f32 = cfg.fp == precision.f32
+        elems = math.prod(int(v) for v in cfg.shape[:cfg.dim + 2] if v)
+        elems = elems // int(cfg.shape[1]) * (int(cfg.shape[1]) // 2 + 1)  # complex side
+        index = (f"(unsigned)offset % {period}u" if elems < (1 << 32)
+                 else f"offset % {period}ul")
+        src = _FILTER_SOURCE.format(t="float" if f32 else "double",
+                                    t2="float2" if f32 else "double2", addr=addr,
+                                    index=index, scale=float(scale).hex(),
+                                    has_filter="true" if filt is not None else "false")
+        c = configuration(cfg.dim, list(cfg.shape), cfg.fp, cfg.dir, cfg.type,
+                          list(cfg.istride), list(cfg.ostride),
+                          user_module(src, None if adjoint else "bbfft_filter_load",
+                                      "bbfft_filter_store" if adjoint else None,
+                                      kernel_language.hip))
+        p = (make_adjoint_plan if adjoint else make_plan)(c, stream=stream,
+                                                          cache=default_cache())
+        _plans[key] = p
+    return p
+
+
+class _FilteredC2R(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, X, filt, scale, cfg, out_shape):
+        X = X.contiguous()
+        y = torch.empty(out_shape, dtype=_real_of(cfg.fp), device=X.device)
+        _filter_plan(cfg, filt, scale, False).execute(X, y)
+        ctx.cfg, ctx.scale = cfg, scale
+        ctx.in_shape, ctx.in_dtype = X.shape, X.dtype
+        ctx.filt = filt  # kept alive: its address is part of the kernels
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        gy = gy.contiguous()
+        gX = torch.empty(ctx.in_shape, dtype=ctx.in_dtype, device=gy.device)
+        _filter_plan(ctx.cfg, ctx.filt, ctx.scale, True).execute(gy, gX)
+        return gX, None, None, None, None

@@ -156,9 +156,14 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     if (!explicit_tuning && info.type == device_type::gpu && info.max_subgroup_size() == 64) {
         gen::lookup_tuning(cfg.fp, cfg.type, N, M, all_flags, tuning);
         // The table is measured on plain kernels; the load-first tile loop needs direct global
-        // loads, so a plan with callbacks keeps the entry's tiling without it.
+        // loads, so a plan with callbacks keeps the entry's tiling without it, and without the
+        // entry's occupancy target and persistent grid: the callback's code adds registers (a
+        // filter-multiplying load callback on the 128-point pass of 32 x 128^3 spilled 46
+        // VGPRs at 4 waves per SIMD and needs 146 without the cap, no spills).
         if (cfg.callbacks) {
             tuning.flags &= ~static_cast<unsigned>(fft1d_prefetch);
+            tuning.waves_per_eu = 0;
+            tuning.groups_per_cu = 0;
         }
     }
 

@@ -271,12 +271,19 @@ std::shared_ptr<plan_base> create_adjoint(configuration const &cfg, hipStream_t 
     if (cfg.dim < 1 || cfg.dim > max_fft_dim) {
         throw bad_configuration("Unsupported FFT dimension: " + std::to_string(cfg.dim));
     }
-    if (cfg.callbacks) {
-        throw bad_configuration("Adjoint plans do not support callbacks");
+    if (cfg.callbacks && (cfg.dim == 1 && cfg.type != transform_type::c2c)) {
+        // the real pass of a 1-D real adjoint carries the fold weights; callbacks are taken
+        // where they run in a c2c pass (c2c plans, and the first or last pass of 2-D/3-D plans)
+        throw bad_configuration("Adjoint plans of 1-D real transforms do not support callbacks");
     }
     unsigned variant = 0;
     auto adj = adjoint_configuration(cfg, &variant);
     auto layout = decompose(adj);
+    for (auto const &p : layout.passes) {
+        if (p.callbacks && p.type != transform_type::c2c) {
+            throw bad_configuration("Adjoint plans take callbacks in c2c passes only");
+        }
+    }
     int device = 0;
     BBFFT_HIP_CHECK(hipGetDevice(&device));
     if (layout.passes.size() == 1) {

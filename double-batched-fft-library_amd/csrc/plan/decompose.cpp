@@ -43,9 +43,6 @@ pass_layout decompose(configuration const &cfg) {
         out.passes.push_back(c);
         return out;
     }
-    if (cfg.callbacks) {
-        throw bad_configuration("User modules are unsupported for FFT dimension > 1.");
-    }
     auto is_default = [&](bool inplace) {
         return same_strides(cfg.istride, default_istride(cfg.dim, cfg.shape, cfg.type, inplace),
                             cfg.dim) &&
@@ -92,6 +89,21 @@ pass_layout decompose(configuration const &cfg) {
         }
     } else {
         out.passes = std::move(p1d);
+    }
+    if (cfg.callbacks) {
+        // Extension (the reference takes callbacks on 1-D plans only): the load callback runs
+        // in the first pass, which reads the plan's input, and the store callback in the last
+        // pass, which writes the plan's output, so offsets index the plan's input and output
+        // tensors as in a 1-D plan. Passes with callbacks are never fused.
+        user_module load = cfg.callbacks, store = cfg.callbacks;
+        load.store_function = nullptr;
+        store.load_function = nullptr;
+        if (load) {
+            out.passes.front().callbacks = load;
+        }
+        if (store) {
+            out.passes.back().callbacks = store;
+        }
     }
     std::size_t const rb = static_cast<std::size_t>(cfg.fp);
     std::size_t const ib = cfg.type == transform_type::r2c ? rb : 2 * rb;
@@ -141,8 +153,8 @@ bool find_fused_pair(pass_layout const &layout, device_info const &info,
 }
 
 configuration adjoint_configuration(configuration const &cfg, unsigned *real_pass_variant) {
+    // the configuration's callbacks (if any) apply to the adjoint plan's own input and output
     configuration adj = cfg;
-    adj.callbacks = {};
     std::swap(adj.istride, adj.ostride);
     switch (cfg.type) {
     case transform_type::c2c:

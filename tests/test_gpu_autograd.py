@@ -158,3 +158,67 @@ def test_3d_backward_forward_time_ratio():
     ratio = mb / mf
     print(f"3-D filter pipeline: forward {mf:.3f} ms, backward {mb:.3f} ms, ratio {ratio:.3f}")
     assert ratio <= 1.2
+
+
+def test_filtered_irfftn_gradcheck():
+    """irfftn with the spectral product and scale fused into its first pass (load callback)
+    and conj(filter) * scale into its adjoint's last pass (store callback): FP64 gradcheck."""
+    shape, n1 = (2, 4, 6, 5), 8
+    X = torch.randn(2, 4, 6, n1 // 2 + 1, dtype=torch.complex128, device="cuda",
+                    requires_grad=True)
+    F = torch.randn(4, 6, n1 // 2 + 1, dtype=torch.complex128, device="cuda")
+    assert torch.autograd.gradcheck(lambda v: ag.irfftn(v, n1, 3, filter=F, scale=0.25),
+                                    (X,), eps=1e-6, atol=1e-8, rtol=1e-6)
+    Y = torch.randn(3, 10, 7, dtype=torch.complex128, device="cuda", requires_grad=True)
+    assert torch.autograd.gradcheck(lambda v: ag.irfftn(v, 12, 2, scale=3.0), (Y,), eps=1e-6,
+                                    atol=1e-8, rtol=1e-6)
+
+
+def _pipeline_fused(x, f, n=128):
+    z = ag.irfftn(ag.rfftn(x, 3), n, 3, filter=f, scale=1.0 / n ** 3)
+    return (z * z).sum(), z
+
+
+def test_3d_filter_pipeline_fused_matches_composition():
+    """32 x 128^3 fp32: the fused spectral product gives the composition's values and
+    gradients (the product and scale are rounded in another order)."""
+    n, B = 128, 32
+    g = torch.Generator(device="cuda")
+    g.manual_seed(13)
+    x = torch.randn(B, n, n, n, generator=g, device="cuda", dtype=torch.float32,
+                    requires_grad=True)
+    f = _filter(n)
+    loss_a, z_a = _pipeline(x, f)
+    loss_b, z_b = _pipeline_fused(x, f)
+    rel = (torch.linalg.vector_norm(z_a - z_b) / torch.linalg.vector_norm(z_a)).item()
+    assert rel < 1e-6, rel
+    ga, = torch.autograd.grad(loss_a, x)
+    gb, = torch.autograd.grad(loss_b, x)
+    rel = (torch.linalg.vector_norm(ga - gb) / torch.linalg.vector_norm(ga)).item()
+    assert rel < 1e-6, rel
+
+
+def test_3d_fused_backward_forward_time_ratio():
+    """The fused pipeline also keeps backward within 1.2x of forward (median of 20)."""
+    n, B = 128, 32
+    x = torch.randn(B, n, n, n, device="cuda", dtype=torch.float32, requires_grad=True)
+    f = _filter(n)
+    for _ in range(3):
+        loss, _ = _pipeline_fused(x, f)
+        torch.autograd.grad(loss, x)
+    torch.cuda.synchronize()
+    tf, tb = [], []
+    for _ in range(20):
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        loss, _ = _pipeline_fused(x, f)
+        e1.record()
+        torch.autograd.grad(loss, x)
+        e2.record()
+        torch.cuda.synchronize()
+        tf.append(e0.elapsed_time(e1))
+        tb.append(e1.elapsed_time(e2))
+    mf, mb = float(np.median(tf)), float(np.median(tb))
+    print(f"fused 3-D filter pipeline: forward {mf:.3f} ms, backward {mb:.3f} ms, "
+          f"ratio {mb / mf:.3f}")
+    assert mb / mf <= 1.2

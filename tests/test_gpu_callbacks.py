@@ -129,3 +129,82 @@ def test_store_callback_truncate_scale_against_oracle(fp, M, N):
     ref = oracle.fft(1, [M, N, K], istride, ostride, 1, -1, xr).reshape(K, n_ref, M)
     got = execute(cfg, x)[:M * n_cut * K].reshape(K, n_cut, M)
     assert oracle.rel_l2(got, ref[:, :n_cut, :] * scale) <= oracle.tolerance(fp, N)
+
+
+_ND_SCALE_SRC = """
+__device__ {t2} ld({t2}* in, size_t offset) {{
+    {t2} v = in[offset]; v.x *= ({t})0.125; v.y *= ({t})0.125; return v;
+}}
+__device__ {t} ldr({t}* in, size_t offset) {{ return in[offset] * ({t})0.125; }}
+__device__ void st({t2}* out, size_t offset, {t2} v) {{
+    v.x *= ({t})4; v.y *= ({t})4; out[offset] = v;
+}}
+__device__ void str_({t}* out, size_t offset, {t} v) {{ out[offset] = v * ({t})4; }}
+"""
+
+
+def _nd_callbacks(cfg, load=True, store=True):
+    t = int(cfg.type)
+    src = _ND_SCALE_SRC.format(t=_real_t(int(cfg.fp)), t2=_real_t(int(cfg.fp)) + "2")
+    return bb.user_module(src, ("ldr" if t == 1 else "ld") if load else None,
+                          ("str_" if t == 2 else "st") if store else None,
+                          bb.kernel_language.hip)
+
+
+def _nd_input(cfg, seed):
+    t = int(cfg.type)
+    n = int(np.prod([cfg.shape[d] for d in range(cfg.dim + 2)]))
+    if t == 2:
+        n = n // cfg.shape[1] * (cfg.shape[1] // 2 + 1)
+    return oracle.uniform_real(seed, n) if t == 1 else oracle.uniform_complex(seed, n)
+
+
+@pytest.mark.parametrize("desc,load,store,fused", [
+    ("scfo16x8x4*3", True, True, False), ("srfo16x8x4*3", True, True, False),
+    ("srbo16x8x4*3", True, True, False), ("dcbo2.12x10*5", True, True, False),
+    # the callback outside the fused pair of passes (r2c: last pass, c2r: first pass)
+    ("srfo64x64x64*2", False, True, True), ("srbo64x64x64*2", True, False, True)])
+def test_nd_callbacks(desc, load, store, fused, monkeypatch):
+    """Callbacks on 2-D/3-D plans (extension): load in the first pass, store in the last, each
+    with the plan's own input / output offsets; the passes without callbacks still fuse.
+    Scaling by powers of two is exact: the result equals the plain plan's (with the same
+    passes, fused or not) on input / 8, times 4."""
+    if not fused:
+        monkeypatch.setenv("BBFFT_FUSE", "off")  # the reference plan's passes, unfused too
+    cfg = bb.parse_fft_descriptor(desc)
+    plain = bb.parse_fft_descriptor(desc)
+    cfg.callbacks = _nd_callbacks(cfg, load, store)
+    x = _nd_input(cfg, 31)
+    y = execute(cfg, x)
+    y_ref = execute(plain, x * (0.125 if load else 1.0))
+    assert np.array_equal(y, y_ref * (4 if store else 1))
+    if fused:
+        p = bb.make_plan(cfg, cache=bb.default_cache())
+        assert any("fused2" in l["kernel"] for l in p.launches())
+
+
+@pytest.mark.parametrize("desc,load,store,ok", [
+    ("srbo16x8x4*3", False, True, True),    # adjoint: r2c-type, store in its last (c2c) pass
+    ("srfo16x8x4*3", True, False, True),    # adjoint: c2r-type, load in its first (c2c) pass
+    ("scfo16x8x4*3", True, True, True),
+    ("srbo16x8x4*3", True, False, False),   # load would run in the adjoint's real pass
+    ("srfo1.16*3", False, True, False),     # 1-D real adjoint
+])
+def test_adjoint_callbacks(desc, load, store, ok, monkeypatch):
+    """Adjoint plans take the configuration's callbacks on their own input and output where
+    those run in a c2c pass; elsewhere plan creation is refused."""
+    monkeypatch.setenv("BBFFT_FUSE", "off")  # same passes with and without callbacks
+    cfg = bb.parse_fft_descriptor(desc)
+    plain = bb.parse_fft_descriptor(desc)
+    cfg.callbacks = _nd_callbacks(bb.adjoint_configuration(plain), load, store)
+    if not ok:
+        with pytest.raises(bb.BadConfiguration):
+            bb.make_adjoint_plan(cfg, cache=bb.default_cache())
+        return
+    adj = bb.adjoint_configuration(plain)
+    x = _nd_input(adj, 32)
+    p = bb.make_adjoint_plan(cfg, cache=bb.default_cache())
+    p_ref = bb.make_adjoint_plan(plain, cache=bb.default_cache())
+    y = execute(adj, x, plan=p)
+    y_ref = execute(adj, x * (0.125 if load else 1.0), plan=p_ref)
+    assert np.array_equal(y, y_ref * (4 if store else 1))
