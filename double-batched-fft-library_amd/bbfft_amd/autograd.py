@@ -14,6 +14,7 @@ c2r. The c2r forward keeps the reference's conventions (imag of the DC bin ignor
 the Nyquist imaginary part contributes -imag(X[N/2]) to every sample), and its backward is the
 exact adjoint of that map.
 """
+import collections
 import math
 
 import torch
@@ -173,14 +174,23 @@ __device__ void bbfft_filter_store({t2}* out, size_t offset, {t2} v) {{
 """
 
 
+_FILTER_PLANS_MAX = 16
+_filter_plans = collections.OrderedDict()  # least recently used first
+
+
 def _filter_plan(cfg, filt, scale, adjoint):
-    """c2r plan loading X * filt * scale, or its adjoint storing (.) * conj(filt) * scale."""
+    """c2r plan loading X * filt * scale, or its adjoint storing (.) * conj(filt) * scale.
+    Plans are kept per filter buffer (its address is compiled into the kernels): a filter that
+    stays in one buffer (updated in place, if at all) compiles once; the last
+    _FILTER_PLANS_MAX plans are kept."""
     stream = torch.cuda.current_stream().cuda_stream
     addr = filt.data_ptr() if filt is not None else 0
     period = filt.numel() if filt is not None else 1
-    key = (cfg.to_string(), adjoint, stream, "filter", addr, period, scale)
-    p = _plans.get(key)
-    if p is None:
+    key = (cfg.to_string(), adjoint, stream, addr, period, scale)
+    p = _filter_plans.get(key)
+    if p is not None:
+        _filter_plans.move_to_end(key)
+    else:
         f32 = cfg.fp == precision.f32
         elems = math.prod(int(v) for v in cfg.shape[:cfg.dim + 2] if v)
         elems = elems // int(cfg.shape[1]) * (int(cfg.shape[1]) // 2 + 1)  # complex side
@@ -197,7 +207,9 @@ def _filter_plan(cfg, filt, scale, adjoint):
                                       kernel_language.hip))
         p = (make_adjoint_plan if adjoint else make_plan)(c, stream=stream,
                                                           cache=default_cache())
-        _plans[key] = p
+        _filter_plans[key] = p
+        while len(_filter_plans) > _FILTER_PLANS_MAX:
+            _filter_plans.popitem(last=False)
     return p
 
 
