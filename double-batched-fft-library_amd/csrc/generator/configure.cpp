@@ -9,6 +9,7 @@
 #include "tuning_table.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <numeric>
 #include <sstream>
 
@@ -164,6 +165,9 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
             tuning.flags &= ~static_cast<unsigned>(fft1d_prefetch);
             tuning.waves_per_eu = 0;
             tuning.groups_per_cu = 0;
+            if (char const *w = std::getenv("BBFFT_CALLBACK_WAVES")) { // A/B
+                tuning.waves_per_eu = static_cast<unsigned>(std::atoi(w));
+            }
         }
     }
 

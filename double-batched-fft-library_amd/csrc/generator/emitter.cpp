@@ -339,15 +339,16 @@ class kernel_gen {
             }
         }
         np_ = (c.lds_pad ? (nfft_ - 1) + (nfft_ - 1) / c.lds_pad : nfft_ - 1) + 1;
-        // 32-bit offsets within a tile when no callback needs absolute offsets and the tile's
-        // extent fits comfortably
+        // 32-bit offsets within a tile when the tile's extent fits comfortably; callbacks,
+        // which take absolute offsets, get the tile's uniform 64-bit base plus the 32-bit
+        // offset within the tile
         std::size_t const kspan = (pairs_ ? 2 : 1) * c.Kb + 1;
         std::size_t const nspan = c.N + 2;
         std::size_t const ext_in = c.Mb + nspan * c.istride[1] + kspan * c.istride[2];
         std::size_t const ext_out = c.Mb + nspan * c.ostride[1] + kspan * c.ostride[2];
         std::size_t const eb = 2 * static_cast<std::size_t>(c.fp); // largest element size
-        local32_ = !c.load_function && !c.store_function &&
-                   ext_in * eb < (std::size_t(1) << 31) && ext_out * eb < (std::size_t(1) << 31);
+        local32_ =
+            ext_in * eb < (std::size_t(1) << 31) && ext_out * eb < (std::size_t(1) << 31);
         half_in_ = (c.flags & (fft1d_input_f16 | fft1d_input_bf16)) != 0;
         bf16_ = (c.flags & fft1d_input_bf16) != 0;
         mfma_ = (c.flags & fft1d_mfma) != 0;
@@ -424,7 +425,8 @@ class kernel_gen {
     /// Global load of input element at offset expression `off` (elements from `in`).
     std::string gload(std::string const &off) const {
         if (c_.load_function) {
-            return std::string(c_.load_function) + "((" + in_type() + "*)in, " + off + ")";
+            std::string const abs = local32_ ? "ibase + (unsigned long)(" + off + ")" : off;
+            return std::string(c_.load_function) + "((" + in_type() + "*)in, " + abs + ")";
         }
         if (c_.flags & fft1d_nontemporal_load) {
             return "ld_nt(tin + " + off + ")";
@@ -434,7 +436,8 @@ class kernel_gen {
     /// Global store of `val` at offset `off` (elements from `out`).
     std::string gstore(std::string const &off, std::string const &val) const {
         if (c_.store_function) {
-            return std::string(c_.store_function) + "(out, " + off + ", " + val + ");";
+            std::string const abs = local32_ ? "obase + (unsigned long)(" + off + ")" : off;
+            return std::string(c_.store_function) + "(out, " + abs + ", " + val + ");";
         }
         if (c_.flags & fft1d_nontemporal_store) {
             return "st_nt(tout + " + off + ", " + val + ");";
@@ -761,6 +764,14 @@ class kernel_gen {
             std::string const kl2 = pairs_ ? "2u * kl" : "kl";
             w_.line("const unsigned ib = ml + " + kl2 + " * " + u(c_.istride[2]) + ";");
             w_.line("const unsigned ob = ml + " + kl2 + " * " + u(c_.ostride[2]) + ";");
+            if (c_.load_function) {
+                w_.line("const unsigned long ibase = m0 + " + kf + "k0 * " +
+                        ul(c_.istride[2]) + "; // tile origin (callback offsets)");
+            }
+            if (c_.store_function) {
+                w_.line("const unsigned long obase = m0 + " + kf + "k0 * " +
+                        ul(c_.ostride[2]) + "; // tile origin (callback offsets)");
+            }
         } else {
             w_.line("const " + in_type() + "* tin = in;");
             w_.line(out_type() + "* tout = out;");
