@@ -378,7 +378,32 @@ def cpu_baseline(budget_s=15.0):
         import oracle.cpu_port as port
     except Exception:  # noqa: BLE001 - the baseline is optional context
         return None
-    return port.benchmark(budget_s=budget_s)
+    res = port.benchmark(budget_s=budget_s)
+    res["scipy_pocketfft"] = scipy_comparator(res.get("cores") or os.cpu_count())
+    return res
+
+
+def scipy_comparator(workers, reps=10):
+    """Non-reference comparator on the same CPU configuration: scipy.fft (pocketfft) c2c
+    forward fp32 over M=1 x N=64 x K=16384 (K rows of 64), min of `reps` runs."""
+    try:
+        import numpy as np
+        import scipy.fft
+    except Exception:  # noqa: BLE001 - optional context
+        return None
+    N, K = 64, 16384
+    rng = np.random.default_rng(1)
+    x = (rng.uniform(-1, 1, (K, N)) + 1j * rng.uniform(-1, 1, (K, N))).astype(np.complex64)
+    scipy.fft.fft(x, axis=1, workers=workers)
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        scipy.fft.fft(x, axis=1, workers=workers)
+        best = min(best, time.perf_counter() - t0)
+    flops = 5.0 * N * math.log2(N) * K
+    return {"GFLOPs": round(flops / best / 1e9, 3), "ms": round(best * 1e3, 4),
+            "workers": workers,
+            "note": "scipy.fft (pocketfft), not the reference algorithm; labelled comparator"}
 
 
 def main():
