@@ -78,12 +78,23 @@ def time_plan(bb, torch, desc, steps, warmup, cache):
     stop.record()
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / steps
+    # per-launch times as the reference harness reports them (min and median of 20
+    # individually event-timed launches, after the back-to-back timed region)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(20)]
+    for i, (a, b) in enumerate(ev):
+        a.record()
+        plan.execute(xs[i % P], ys[i % P])
+        b.record()
+    torch.cuda.synchronize()
+    per = sorted(a.elapsed_time(b) for a, b in ev)
     flops = 5.0 * N * math.log2(N) * M * K
     launch = plan.launches()[0]
     del xs, ys
     return {"desc": desc, "ms": ms, "gflops": flops / ms / 1e6, "GBps": nbytes / ms / 1e6,
             "frac": nbytes / ms / 1e6 / HBM_PEAK_GBPS, "bytes": nbytes, "kernel": launch,
-            "buffer_pairs": P}
+            "buffer_pairs": P, "launch_min_ms": per[0],
+            "launch_median_ms": 0.5 * (per[9] + per[10])}
 
 
 def torch_fft_comparator(torch, desc, steps):
@@ -446,6 +457,8 @@ def main():
                 sweep.append({"desc": r["desc"], "ms": round(r["ms"], 5),
                               "GFLOPs": round(r["gflops"], 1), "GBps": round(r["GBps"], 1),
                               "frac": round(r["frac"], 4), "buffer_pairs": r["buffer_pairs"],
+                              "launch_min_ms": round(r["launch_min_ms"], 5),
+                              "launch_median_ms": round(r["launch_median_ms"], 5),
                               "torch_fft_GBps": v["GBps"]})
                 print(json.dumps(sweep[-1]), file=sys.stderr, flush=True)
     nd = None
@@ -489,6 +502,8 @@ def main():
                    "frac_of_stream_copy": (round(head["GBps"] / max(stream.values()), 4)
                                            if stream else None),
                    "kernel": head["kernel"]["kernel"],
+                   "launch_min_ms": round(head["launch_min_ms"], 6),
+                   "launch_median_ms": round(head["launch_median_ms"], 6),
                    "torch_fft_same_tensor": vendor,
                    "sweep": sweep, "nd_real_3d_32x128cubed": nd,
                    "autograd_3d_filter_32x128cubed": grad, "mfma_split_n512": mf,
