@@ -118,7 +118,10 @@ def irfftn(X, n1, ndim=1, filter=None, scale=None):
     `scale` (a number) it computes irfftn(X * filter) * scale, the product taken by the
     transform's first pass as it loads X (a load callback) instead of a separate pass over the
     spectrum; its backward multiplies by conj(filter) * scale as the adjoint's last pass stores
-    (a store callback). The filter is a constant: it gets no gradient."""
+    (a store callback). The filter is a constant: it gets no gradient. Its buffer's address is
+    compiled into the kernels (values are read at each call), so keep one contiguous filter
+    tensor across calls: a new buffer (or a non-contiguous filter, which is copied) compiles
+    new kernels."""
     if not X.is_complex():
         raise TypeError("irfftn expects a complex tensor")
     if X.shape[-1] != n1 // 2 + 1:
