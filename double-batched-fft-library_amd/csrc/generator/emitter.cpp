@@ -1436,12 +1436,23 @@ class kernel_gen {
         std::size_t const Q = half_ ? H / 2 + 1 : (c_.N - 1) / 2 + 1;
         std::size_t const B = (Q + c_.Nt - 1) / c_.Nt;
         std::string const half = f_.lit(0.5);
+        // When the last round holds only the self-conjugate q = H/2 (lane t = 0), that round is
+        // X[H/2] = conj(z[H/2]): the general formula with the table's twiddle (1, 0) gives the
+        // same bits, at the cost of a full round for one lane.
+        bool const lone_mid = half_ && !adjoint() && H % 2 == 0 && (Q - 1) % c_.Nt == 0;
         // phase 1: read the complex FFT output and form the two spectrum values per q
         for (std::size_t b = 0; b < B; ++b) {
             std::string q = "q" + std::to_string(b);
             std::string o1 = "o1_" + std::to_string(b), o2 = "o2_" + std::to_string(b);
             w_.line("const unsigned " + q + " = t + " + u(b * c_.Nt) + ";");
             w_.line(T2() + " " + o1 + " = " + f_.zero2() + ", " + o2 + " = " + f_.zero2() + ";");
+            if (lone_mid && b + 1 == B) {
+                w_.open("if (active && " + q + " == " + u(H / 2) + ")");
+                w_.line("const " + T2() + " y = " + lds(u(H / 2)) + ";");
+                w_.line(o1 + " = mk(y.x, -y.y);");
+                w_.close();
+                continue;
+            }
             w_.open("if (active && " + q + " < " + u(Q) + ")");
             if (half_) {
                 w_.line("const " + T2() + " y1 = " + lds(q + " % " + u(H)) + ";");
