@@ -1368,12 +1368,23 @@ class kernel_gen {
         std::size_t const H = nfft_;
         std::size_t const Q = half_ ? H / 2 + 1 : (c_.N - 1) / 2 + 1;
         std::size_t const B = (Q + c_.Nt - 1) / c_.Nt;
+        // When the last round holds only the self-conjugate q = H/2 (lane t = 0), its input is
+        // 2 conj(X[H/2]): the general formula with the table's twiddle (-1, -0) gives the same
+        // values (zeros may differ in sign), at the cost of a full round for one lane.
+        bool const lone_mid = half_ && !adjoint() && H % 2 == 0 && (Q - 1) % c_.Nt == 0;
         // phase 1: read the half spectrum and form the complex FFT inputs
         for (std::size_t b = 0; b < B; ++b) {
             std::string q = "q" + std::to_string(b);
             std::string s1 = "s1_" + std::to_string(b), s2 = "s2_" + std::to_string(b);
             w_.line("const unsigned " + q + " = t + " + u(b * c_.Nt) + ";");
             w_.line(T2() + " " + s1 + " = " + f_.zero2() + ", " + s2 + " = " + f_.zero2() + ";");
+            if (lone_mid && b + 1 == B) {
+                w_.open("if (active && " + q + " == " + u(H / 2) + ")");
+                w_.line("const " + T2() + " x = " + gload(ioff(u(H / 2))) + ";");
+                w_.line(s1 + " = mk(x.x + x.x, -(x.y + x.y));");
+                w_.close();
+                continue;
+            }
             w_.open("if (active && " + q + " < " + u(Q) + ")");
             if (half_) {
                 w_.line(T2() + " x1 = " + gload(ioff(q)) + ";");
