@@ -27,6 +27,9 @@ enum class kernel_language {
  * The load function has the signature `T2 load(T2* in, size_t offset)` (or `T load(T* in, size_t
  * offset)` for real input) and the store function `void store(T2* out, size_t offset, T2 value)`
  * (or with T for real output). Offsets are in elements from the tensor base pointer.
+ * On 2-D/3-D plans (an extension: the reference accepts callbacks on 1-D plans only) the load
+ * callback runs in the plan's first pass and the store callback in its last, with offsets into
+ * the plan's input and output tensors; adjoint plans take them where they run in a c2c pass.
  */
 struct BBFFT_EXPORT user_module {
     char const *data = nullptr;           ///< Source code string

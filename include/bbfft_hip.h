@@ -167,7 +167,10 @@ BBFFT_EXPORT_C int bbfft_plan_create_from_descriptor(const char *desc, bbfft_str
  * No reference counterpart (an extension): c2c -> opposite-sign unnormalised transform with
  * in/out layouts swapped; r2c -> Hermitian-fold weighting + c2r; c2r -> r2c + doubling of the
  * non-self-conjugate bins. bbfft_adjoint_configuration returns the adjoint's configuration
- * (its real pass additionally runs the weighting variant). */
+ * (its real pass additionally runs the weighting variant). Callbacks of cfg apply to the
+ * adjoint plan's own input (load) and output (store) where they run in a c2c pass (1-D c2c, the
+ * first or last pass of 2-D/3-D plans); otherwise plan creation fails with
+ * BBFFT_ERROR_BAD_CONFIGURATION. */
 BBFFT_EXPORT_C int bbfft_adjoint_configuration(const bbfft_configuration *cfg,
                                                bbfft_configuration *adjoint);
 BBFFT_EXPORT_C int bbfft_plan_create_adjoint(const bbfft_configuration *cfg,
