@@ -323,7 +323,7 @@ class kernel_gen {
         : c_(c), f_{c.fp}, name_(std::move(name)), group_size_(group_size), alias_in_(alias_in),
           alias_out_(alias_out), lds_tables_(opt.lds_tables), lds_barriers_(opt.lds_barriers),
           extra_param_(opt.extra_param), after_input_(opt.after_input),
-          input_index_xor_(opt.input_index_xor) {
+          input_index_xor_(opt.input_index_xor), single_tile_(opt.single_tile) {
         nfft_ = c.N_fft();
         L_ = c.factorization.size();
         real_ = c.type != transform_type::c2c;
@@ -746,7 +746,12 @@ class kernel_gen {
         if (MG_ * c_.Mb != c_.M) {
             act = "m < " + ul(c_.M) + " && " + act;
         }
-        if (!band_.empty()) {
+        if (single_tile_) {
+            if (MG_ * c_.Mb != c_.M) {
+                throw bad_configuration("a single-tile body needs Mb = M");
+            }
+            act = band_.empty() ? "true" : band_;
+        } else if (!band_.empty()) {
             act = band_ + " && " + act;
         }
         w_.line("const bool active = " + act + ";");
@@ -1024,12 +1029,17 @@ class kernel_gen {
                 }
                 continue;
             }
-            std::string cond = guard[b];
-            if (src_global) {
-                cond = cond.empty() ? "active" : "active && " + cond;
-            } else if (!band_.empty()) {
-                cond = cond.empty() ? band_ : band_ + " && " + cond;
+            // Condition of the butterfly's computation (below) and of its loads: work-items
+            // that compute without loading (no column, inside the band) hold zeros.
+            std::string ccond = guard[b];
+            if (!band_.empty()) {
+                ccond = ccond.empty() ? band_ : band_ + " && " + ccond;
             }
+            std::string cond = ccond;
+            if (src_global && !single_tile_) {
+                cond = guard[b].empty() ? "active" : "active && " + guard[b];
+            }
+            bool const zero_fill = cond != ccond;
             if (!cond.empty()) {
                 w_.open("if (" + cond + ")");
             }
@@ -1060,14 +1070,15 @@ class kernel_gen {
                 }
             }
             if (!cond.empty()) {
-                // Work-items without a butterfly (or without a column) hold zeros; the branch
-                // is skipped entirely when a whole wave is active.
                 w_.close();
-                w_.open("else");
-                for (std::size_t i = 0; i < R; ++i) {
-                    w_.line(v[b][i] + " = " + f_.zero2() + ";");
+                if (zero_fill) {
+                    // the branch is skipped entirely when a whole wave is active
+                    w_.open("else");
+                    for (std::size_t i = 0; i < R; ++i) {
+                        w_.line(v[b][i] + " = " + f_.zero2() + ";");
+                    }
+                    w_.close();
                 }
-                w_.close();
             }
         }
         if (src_global && prefetch_) {
@@ -1541,6 +1552,7 @@ class kernel_gen {
     std::string extra_param_;  // extra parameter of a tile body
     std::string after_input_;  // code run once stage 0 has read all of the tile's input
     std::string input_index_xor_; // XOR-ed into input element indices (permuted input rows)
+    bool single_tile_ = false;    // one tile, all columns: `active` is the band alone
     int direct_ = 0; // radix of the direct DFT stage (0: none)
     bool local32_ = false;
     bool prefetch_ = false;

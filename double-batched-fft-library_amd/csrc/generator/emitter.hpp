@@ -30,6 +30,10 @@ struct body_options {
     /// Expression XOR-ed into the element index of every input read (c2c, or r2c with
     /// even N read as complex pairs): the input sits in a row-wise permuted layout.
     std::string input_index_xor;
+    /// The body is always called with tile 0 and K equal to its column tile (one tile covers
+    /// all columns, M = Mb): every work-item of the group's band has a column, so `active` is
+    /// the band alone and work-items outside it neither load nor compute.
+    bool single_tile = false;
 };
 /// Tile body whose input (alias_in) or output (alias_out) pointer may point into the LDS
 /// scratch passed to the body (real transforms included).

@@ -241,12 +241,17 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
                                           option("BBFFT_FUSE_DMA", "16", true)
                                       ? 16
                                       : 4;
-            // staging loads from the waves idle in pass b (BBFFT_FUSE_DMAWAVES=split)
+            // staging loads from the waves idle in pass b (BBFFT_FUSE_DMAWAVES=split) or, with
+            // the early copy, from those idle in pass a after its first stage (=idle_a)
+            std::size_t const a_waves = (st.a.work_group_size() + 63) / 64;
             std::size_t const b_waves = (st.b.work_group_size() + 63) / 64;
-            if (option("BBFFT_FUSE_DMAWAVES", "split", r2c_first) && st.lds_barriers &&
-                st.lds_tables &&
-                st.group_size % 64 == 0 && b_waves < st.group_size / 64) {
+            std::string const dma_waves = env("BBFFT_FUSE_DMAWAVES");
+            bool const lds_only = st.lds_barriers && st.lds_tables && st.group_size % 64 == 0;
+            if ((dma_waves.empty() ? r2c_first : dma_waves == "split") && lds_only &&
+                b_waves < st.group_size / 64) {
                 st.copy_wave0 = static_cast<unsigned>(b_waves);
+            } else if (dma_waves == "idle_a" && lds_only && a_waves < st.group_size / 64) {
+                st.copy_wave0 = static_cast<unsigned>(a_waves);
             }
             // next plane's copy issued right after pass a's first stage (BBFFT_FUSE_EARLY=on);
             // only useful when barriers do not wait for it
@@ -384,6 +389,7 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
     body_options oa, ob;
     oa.lds_tables = ob.lds_tables = fc.lds_tables;
     oa.lds_barriers = ob.lds_barriers = fc.lds_barriers;
+    oa.single_tile = ob.single_tile = true; // one tile per plane: Kb = K, Mb = M
     std::string const sync = fc.lds_barriers ? "BBFFT_LDS_SYNC();" : "__syncthreads();";
     if (fc.stage_words && fc.early_copy) {
         oa.extra_param = "const float* next_input, float* stage";

@@ -191,6 +191,34 @@ def test_fused_swizzled_stage_matches_plain(t, M, dims, K, monkeypatch):
     assert oracle.rel_l2(ys[0], ref) <= oracle.tolerance(4, int(np.prod(dims)))
 
 
+def test_fused_copy_by_waves_idle_in_first_pass(monkeypatch):
+    """c2r plane whose first (c2c) pass needs fewer waves than its second: with
+    BBFFT_FUSE_DMAWAVES=idle_a the next plane's staging loads are issued right after the first
+    pass's first stage by the waves that have no work in the rest of that pass. Same bits as
+    the plain kernel and as the unstaged one."""
+    cfg = _cfg(1, [32, 64], 3, 4, 2)
+    half = list(cfg.shape)
+    half[1] = 32 // 2 + 1
+    n_in = oracle.extent(2, half, cfg.istride)
+    n_out = oracle.extent(2, cfg.shape, cfg.ostride)
+    x = oracle.uniform_complex(8, n_in).astype(np.complex64)
+    monkeypatch.setenv("BBFFT_FUSE_WGMAX", "256")  # pass a: 136 work-items, pass b: 256
+    ys = []
+    for env in (PLAIN, {"BBFFT_FUSE_SYNC": "lds", "BBFFT_FUSE_TABLES": "lds",
+                        "BBFFT_FUSE_EARLY": "on", "BBFFT_FUSE_DMAWAVES": "idle_a"},
+                {"BBFFT_FUSE_STAGE": "off"}):
+        for k in list(PLAIN) + ["BBFFT_FUSE_STAGE"]:
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        src = bb.generate_fused_source(cfg)
+        if env.get("BBFFT_FUSE_DMAWAVES") == "idle_a":
+            assert "threadIdx.x / 64u - 3u < 1u) BBFFT_WAIT_ALL(); // copying waves" in src
+            assert "(next_input, stage);" in src
+        ys.append(_run_plan(cfg, x, n_out))
+    assert all(np.array_equal(ys[0], y) for y in ys[1:])
+
+
 def test_fused_in_place_layout_r2c():
     """In-place r2c layout (padded real rows) read by the fused kernel."""
     cfg = _cfg(1, [16, 8, 4], 2, 4, 1, inplace=True)

@@ -114,6 +114,19 @@ def test_fused_kernels_are_race_free(t, M, dims, K, variant, monkeypatch):
     assert ok, report[-3000:]
 
 
+def test_fused_copy_by_waves_idle_in_first_pass_is_race_free(monkeypatch):
+    """The staging copy issued by the waves idle in the first pass (c2r plane, 136 of 256
+    work-items busy in pass a) lands while the other waves finish that pass."""
+    monkeypatch.setenv("BBFFT_FUSE", "on")
+    for k, v in {"BBFFT_FUSE_WGMAX": "256", "BBFFT_FUSE_SYNC": "lds", "BBFFT_FUSE_TABLES": "lds",
+                 "BBFFT_FUSE_EARLY": "on", "BBFFT_FUSE_DMAWAVES": "idle_a"}.items():
+        monkeypatch.setenv(k, v)
+    args = _fused(1, [32, 64], 3, 2)
+    assert "copying waves" in args[0]
+    ok, report = runner.race_check(*args)
+    assert ok, report[-3000:]
+
+
 def test_ragged_kernel_is_race_free():
     rng = np.random.default_rng(5)
     jobs = ragged.pack([ragged.Job(M=int(rng.integers(1, 4)), N=int(n), K=int(rng.integers(1, 6)),
