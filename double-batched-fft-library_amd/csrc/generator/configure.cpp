@@ -156,6 +156,9 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
     bool const mfma = (all_flags & fft1d_mfma) != 0;
     if (!explicit_tuning && info.type == device_type::gpu && info.max_subgroup_size() == 64) {
         gen::lookup_tuning(cfg.fp, cfg.type, N, M, all_flags, tuning);
+        if (char const *m = std::getenv("BBFFT_TABLE_FLAGS_CLEAR")) { // A/B: drop entry flags
+            tuning.flags &= ~static_cast<unsigned>(std::strtoul(m, nullptr, 0));
+        }
         // The table is measured on plain kernels; the load-first tile loop needs direct global
         // loads, so a plan with callbacks keeps the entry's tiling without it, and without the
         // entry's occupancy target and persistent grid: the callback's code adds registers (a
