@@ -131,7 +131,7 @@ std::string fused2_configuration::identifier() const {
                    ";" + std::to_string(lds_barriers) + ";" + std::to_string(lds_tables) +
                    ";" + std::to_string(waves_per_eu) + ";" + std::to_string(copy_wave0) + ";" +
                    std::to_string(early_copy) + ";" + std::to_string(stage_swizzle_rows) + ";" +
-                   std::to_string(stage_swizzle_elems));
+                   std::to_string(stage_swizzle_elems) + ";" + std::to_string(stage_load_nt));
     return oss.str();
 }
 
@@ -241,6 +241,9 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
                                           option("BBFFT_FUSE_DMA", "16", true)
                                       ? 16
                                       : 4;
+            // Staging loads with the non-temporal cache policy (BBFFT_FUSE_DMA_NT=on); plans
+            // choose it from their other passes' stores (find_fused_pair).
+            st.stage_load_nt = option("BBFFT_FUSE_DMA_NT", "on", false);
             // staging loads from the waves idle in pass b (BBFFT_FUSE_DMAWAVES=split) or, with
             // the early copy, from those idle in pass a after its first stage (=idle_a)
             std::size_t const a_waves = (st.a.work_group_size() + 63) / 64;
@@ -325,6 +328,17 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
                   "(__attribute__((address_space(3))) void*)(dst), 16, 0, 0)\n"
                   "#endif\n";
         }
+        if (fc.stage_load_nt) {
+            // the same loads with the non-temporal cache policy (aux bit 1 on gfx950)
+            os << "#ifndef BBFFT_LDS_DMA4_NT\n"
+                  "#define BBFFT_LDS_DMA4_NT(src, dst) __builtin_amdgcn_global_load_lds("
+                  "(const __attribute__((address_space(1))) void*)(src), "
+                  "(__attribute__((address_space(3))) void*)(dst), 4, 0, 2)\n"
+                  "#define BBFFT_LDS_DMA16_NT(src, dst) __builtin_amdgcn_global_load_lds("
+                  "(const __attribute__((address_space(1))) void*)(src), "
+                  "(__attribute__((address_space(3))) void*)(dst), 16, 0, 2)\n"
+                  "#endif\n";
+        }
     }
     std::size_t const in_bytes = fc.a.type == transform_type::r2c
                                      ? static_cast<std::size_t>(fc.a.fp)
@@ -355,6 +369,7 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
                        std::to_string(4 * CR) + "u) % " + std::to_string(fc.stage_swizzle_rows) +
                        "u) * " + std::to_string(SC) + "u)) * 4u) + i % 4u)";
         }
+        std::string const nt = fc.stage_load_nt ? "_NT" : "";
         os << "// copy of one plane's input (src, or nothing when src is null) into the staging\n"
            << "// buffer `stage` (LDS)\n"
            << "static __device__ __forceinline__ void " << copy
@@ -373,7 +388,7 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
             os << "  if ((reinterpret_cast<unsigned long>(src) & 15ul) == 0ul) {\n"
                << "    for (unsigned b = wv; b < " << NB16 << "u; b += " << NWc << "u) {\n"
                << "      const unsigned i = b * 256u + 4u * ln;\n"
-               << "      if (i < " << W << "u) BBFFT_LDS_DMA16(src + " << src_word
+               << "      if (i < " << W << "u) " << "BBFFT_LDS_DMA16" << nt << "(src + " << src_word
                << ", stage + b * 256u);\n"
                << "    }\n"
                << "    return;\n"
@@ -381,7 +396,7 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
         }
         os << "  for (unsigned b = wv; b < " << NB << "u; b += " << NWc << "u) {\n"
            << "    const unsigned i = b * " << BS << "u + ln;\n"
-           << "    if (i < " << W << "u) BBFFT_LDS_DMA4(src + " << src_word << ", stage + b * "
+           << "    if (i < " << W << "u) " << "BBFFT_LDS_DMA4" << nt << "(src + " << src_word << ", stage + b * "
            << BS << "u);\n"
            << "  }\n"
            << "}\n";

@@ -148,6 +148,34 @@ bool find_fused_pair(pass_layout const &layout, device_info const &info,
     if (!forced && !small && (fc.lds_bytes() > 96 * 1024 || fc.planes < 256)) {
         return false;
     }
+    // Staging loads with the non-temporal cache policy when every other pass of the plan
+    // writes with plain stores: those leave dirty lines in the 256 MB last-level cache, which
+    // cache-allocating staging loads would evict (and write back) during the plane kernel.
+    // Measured on MI355X (bench.py's plans, processes alternated): 32 x 128^3 fp32 r2c
+    // 0.1997 ms against 0.2070, c2r 0.2045 against 0.2142 (third pass with plain stores);
+    // plain loads were faster where the other pass streams its stores (c2c 32 x 64 x 128 x 128
+    // 0.2574 against 0.2654, r2c 256 x 64^3 0.2293 against 0.2307) and in 2-D plans, whose
+    // single kernel then re-reads part of a repeated input from that cache.
+    // BBFFT_FUSE_DMA_NT=on|off overrides.
+    char const *nt_env = std::getenv("BBFFT_FUSE_DMA_NT");
+    if (fc.stage_words && !nt_env) {
+        bool plain_stores = D > 2;
+        for (std::size_t i = 0; i < D && plain_stores; ++i) {
+            if (i == first || i == first + 1) {
+                continue;
+            }
+            try {
+                unsigned const v = layout.passes[i].type == transform_type::c2c
+                                       ? 0u
+                                       : real_pass_variant;
+                auto const k = configure_fft1d(layout.passes[i], info, fft1d_tuning{}, v);
+                plain_stores = (k.flags & fft1d_nontemporal_store) == 0u;
+            } catch (bad_configuration const &) {
+                plain_stores = false;
+            }
+        }
+        fc.stage_load_nt = plain_stores;
+    }
     out = fused_pair{first, fc};
     return true;
 }

@@ -168,6 +168,8 @@ struct BBFFT_EXPORT fused2_configuration {
     /// per wave) or 4. 16-byte loads need a 16-byte aligned input, which the kernel checks;
     /// otherwise it takes the 4-byte loads.
     unsigned stage_load_bytes = 4;
+    /// The staging loads carry the non-temporal cache policy (the input is read once).
+    bool stage_load_nt = false;
     /// Barriers order LDS accesses only: each work-item's global accesses within a plane are
     /// its own, so waits for outstanding global loads and stores are not needed at barriers.
     bool lds_barriers = false;

@@ -39,6 +39,9 @@ struct emu_dim3 {
         for (int q_ = 0; q_ < 4; ++q_)                                                          \
             ((float *)(dst))[4u * (threadIdx.x % 64u) + q_] = ((const float *)(src))[q_];      \
     } while (0)
+// the same with the non-temporal cache policy (no effect on the values)
+#define BBFFT_LDS_DMA4_NT(src, dst) BBFFT_LDS_DMA4(src, dst)
+#define BBFFT_LDS_DMA16_NT(src, dst) BBFFT_LDS_DMA16(src, dst)
 #define BBFFT_WAIT_ALL() ((void)0)
 #define BBFFT_LDS_SYNC() __syncthreads()
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))

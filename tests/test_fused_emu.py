@@ -235,6 +235,22 @@ def test_fused_in_place_layout_r2c():
     assert oracle.rel_l2(y[sel], ref[sel]) <= oracle.tolerance(4, 512)
 
 
+def test_staging_cache_policy_follows_the_other_passes(monkeypatch):
+    """The plane kernel's staging loads take the non-temporal cache policy when every other
+    pass of the plan stores plainly (the 32 x 128^3 real plans, whose third pass has a plain
+    store entry in the tuning table), and plain loads when another pass streams its stores or
+    the plan has no other pass; BBFFT_FUSE_DMA_NT overrides."""
+    def nt(desc):
+        return "BBFFT_LDS_DMA16_NT(src" in bb.generate_fused_source(bb.parse_fft_descriptor(desc))
+    assert nt("srfo128x128x128*32") and nt("srbo128x128x128*32")
+    assert not nt("scfo64x128x128*32") and not nt("srfo64x64x64*256")
+    assert not nt("srfo128x128*2048") and not nt("dcfo64x64*1024")
+    monkeypatch.setenv("BBFFT_FUSE_DMA_NT", "on")
+    assert nt("srfo128x128*2048")
+    monkeypatch.setenv("BBFFT_FUSE_DMA_NT", "off")
+    assert not nt("srfo128x128x128*32")
+
+
 def test_fusion_policy(monkeypatch):
     """Planes of at most 96 KB of LDS fuse, among them the 128 x 65 fp32 plane of the
     32 x 128^3 real transforms (70 KB); larger planes (the 138 KB fp32 128 x 128 plane) are left
