@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <string>
 #include <numeric>
 #include <sstream>
 
@@ -170,6 +171,14 @@ fft1d_configuration configure_fft1d(configuration const &cfg, device_info const 
             tuning.groups_per_cu = 0;
             if (char const *w = std::getenv("BBFFT_CALLBACK_WAVES")) { // A/B
                 tuning.waves_per_eu = static_cast<unsigned>(std::atoi(w));
+            }
+            // A kernel with a load callback streams its stores past the last-level cache: the
+            // callback may read side data (a filter read once per batch) that the kernel's
+            // output would otherwise evict. BBFFT_CALLBACK_NT=off keeps the entry's stores.
+            char const *nt = std::getenv("BBFFT_CALLBACK_NT");
+            if (cfg.callbacks.load_function && tuning.has_flags &&
+                !(nt && std::string(nt) == "off")) {
+                tuning.flags |= fft1d_nontemporal_store;
             }
         }
     }
