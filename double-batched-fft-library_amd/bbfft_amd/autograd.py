@@ -150,6 +150,9 @@ def irfftn(X, n1, ndim=1, filter=None, scale=None):
 # offsets index the spectrum, whose trailing modes repeat every `period` elements. The
 # remainder is taken in 32-bit arithmetic when the spectrum has fewer than 2^32 elements (a
 # 64-bit remainder per element costs the 16-element work-items enough registers to spill).
+# The store callback writes with non-temporal stores, as the library does for kernels with a
+# load callback: the output then does not push the filter out of the last-level cache (32 x
+# 128^3 backward 0.765-0.770 ms against 0.773 with plain stores).
 _FILTER_SOURCE = """
 __device__ {t2} bbfft_filter_load({t2}* in, size_t offset) {{
     const {t2}* h = (const {t2}*){addr:#x}ul;
@@ -172,7 +175,8 @@ __device__ void bbfft_filter_store({t2}* out, size_t offset, {t2} v) {{
     {t2} r;
     r.x = (v.x * w.x + v.y * w.y) * ({t}){scale};
     r.y = (v.y * w.x - v.x * w.y) * ({t}){scale};
-    out[offset] = r;
+    __builtin_nontemporal_store(r.x, &out[offset].x);
+    __builtin_nontemporal_store(r.y, &out[offset].y);
 }}
 """
 
