@@ -10,9 +10,9 @@
 
 #include <algorithm>
 #include <cstdlib>
-#include <string>
 #include <numeric>
 #include <sstream>
+#include <string>
 
 namespace bbfft {
 

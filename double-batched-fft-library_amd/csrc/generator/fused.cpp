@@ -370,6 +370,7 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
                        "u) * " + std::to_string(SC) + "u)) * 4u) + i % 4u)";
         }
         std::string const nt = fc.stage_load_nt ? "_NT" : "";
+        std::string const dma16 = "BBFFT_LDS_DMA16" + nt, dma4 = "BBFFT_LDS_DMA4" + nt;
         os << "// copy of one plane's input (src, or nothing when src is null) into the staging\n"
            << "// buffer `stage` (LDS)\n"
            << "static __device__ __forceinline__ void " << copy
@@ -388,7 +389,7 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
             os << "  if ((reinterpret_cast<unsigned long>(src) & 15ul) == 0ul) {\n"
                << "    for (unsigned b = wv; b < " << NB16 << "u; b += " << NWc << "u) {\n"
                << "      const unsigned i = b * 256u + 4u * ln;\n"
-               << "      if (i < " << W << "u) " << "BBFFT_LDS_DMA16" << nt << "(src + " << src_word
+               << "      if (i < " << W << "u) " << dma16 << "(src + " << src_word
                << ", stage + b * 256u);\n"
                << "    }\n"
                << "    return;\n"
@@ -396,8 +397,8 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
         }
         os << "  for (unsigned b = wv; b < " << NB << "u; b += " << NWc << "u) {\n"
            << "    const unsigned i = b * " << BS << "u + ln;\n"
-           << "    if (i < " << W << "u) " << "BBFFT_LDS_DMA4" << nt << "(src + " << src_word << ", stage + b * "
-           << BS << "u);\n"
+           << "    if (i < " << W << "u) " << dma4 << "(src + " << src_word
+           << ", stage + b * " << BS << "u);\n"
            << "  }\n"
            << "}\n";
     }
