@@ -323,7 +323,8 @@ class kernel_gen {
         : c_(c), f_{c.fp}, name_(std::move(name)), group_size_(group_size), alias_in_(alias_in),
           alias_out_(alias_out), lds_tables_(opt.lds_tables), lds_barriers_(opt.lds_barriers),
           extra_param_(opt.extra_param), after_input_(opt.after_input),
-          input_index_xor_(opt.input_index_xor), single_tile_(opt.single_tile) {
+          input_index_xor_(opt.input_index_xor), single_tile_(opt.single_tile),
+          lds_table_pad_(opt.lds_tables && opt.lds_table_pad) {
         nfft_ = c.N_fft();
         L_ = c.factorization.size();
         real_ = c.type != transform_type::c2c;
@@ -453,6 +454,8 @@ class kernel_gen {
         return name_ + "_tw" + std::to_string(s) + (lds_tables_ ? "_l" : "");
     }
     std::string twr_name() const { return name_ + "_twr" + (lds_tables_ ? "_l" : ""); }
+    /// Entries between the rows (j mod p) of a stage's twiddle table of radix R.
+    std::size_t tw_row(std::size_t R) const { return R + (lds_table_pad_ ? 1 : 0); }
     std::string dtab_name() const { return name_ + "_dtab"; }
 
     void emit_table(std::string const &name, std::vector<std::pair<long, long>> const &roots,
@@ -551,24 +554,34 @@ class kernel_gen {
     /// LDS copies of the twiddle tables and `NAME_tables(lid, n)`, which fills them with the
     /// n work-items lid = 0..n-1 (the caller orders it before the first use with a barrier).
     void emit_lds_tables() {
-        std::vector<std::pair<std::string, std::size_t>> tabs;
+        struct table {
+            std::string name;
+            std::size_t entries, row; // row: entries per row of the constant table (0: one row)
+        };
+        std::vector<table> tabs;
         std::size_t p = c_.factorization[0];
         for (std::size_t s = 1; s < L_; ++s) {
-            tabs.push_back({name_ + "_tw" + std::to_string(s), p * c_.factorization[s]});
+            tabs.push_back({name_ + "_tw" + std::to_string(s), p * c_.factorization[s],
+                            c_.factorization[s]});
             p *= c_.factorization[s];
         }
         if (half_) {
-            tabs.push_back({name_ + "_twr", nfft_ / 2 + 1});
+            tabs.push_back({name_ + "_twr", nfft_ / 2 + 1, 0});
         }
-        for (auto const &[t, n] : tabs) {
-            w_.line("__shared__ " + T2() + " " + t + "_l[" + std::to_string(n) + "];");
+        for (auto const &t : tabs) {
+            std::size_t const n = t.row ? t.entries / t.row * tw_row(t.row) : t.entries;
+            w_.line("__shared__ " + T2() + " " + t.name + "_l[" + std::to_string(n) + "];");
         }
         w_.line("static __device__ __forceinline__ void " + name_ +
                 "_tables(unsigned lid, unsigned n) {");
         w_.open("");
-        for (auto const &[t, n] : tabs) {
-            w_.line("for (unsigned q = lid; q < " + u(n) + "; q += n) " + t + "_l[q] = " + t +
-                    "[q];");
+        for (auto const &t : tabs) {
+            std::string dst = "q";
+            if (t.row && tw_row(t.row) != t.row) {
+                dst = "(q / " + u(t.row) + ") * " + u(tw_row(t.row)) + " + q % " + u(t.row);
+            }
+            w_.line("for (unsigned q = lid; q < " + u(t.entries) + "; q += n) " + t.name + "_l[" +
+                    dst + "] = " + t.name + "[q];");
         }
         w_.close();
         w_.line("}");
@@ -1063,7 +1076,7 @@ class kernel_gen {
                 }
             }
             if (s > 0) {
-                std::string jp = "(" + jn[b] + " % " + u(p) + ") * " + u(R);
+                std::string jp = "(" + jn[b] + " % " + u(p) + ") * " + u(tw_row(R));
                 for (std::size_t i = 1; i < R; ++i) {
                     w_.line(v[b][i] + " = c_mul(" + v[b][i] + ", " + tw_name(s) + "[" + jp +
                             " + " + u(i) + "]);");
@@ -1236,7 +1249,7 @@ class kernel_gen {
                 w_.line("x2 = " + lds("dn2") + ";");
             }
             if (s > 0) {
-                std::string jp = "(dj % " + u(p) + ") * " + u(R);
+                std::string jp = "(dj % " + u(p) + ") * " + u(tw_row(R));
                 w_.line("x1 = c_mul(x1, " + tw_name(s) + "[" + jp + " + di]);");
                 w_.line("x2 = c_mul(x2, " + tw_name(s) + "[" + jp + " + (di == 0u ? 0u : " + u(R) +
                         " - di)]);");
@@ -1553,6 +1566,7 @@ class kernel_gen {
     std::string after_input_;  // code run once stage 0 has read all of the tile's input
     std::string input_index_xor_; // XOR-ed into input element indices (permuted input rows)
     bool single_tile_ = false;    // one tile, all columns: `active` is the band alone
+    bool lds_table_pad_ = false;  // LDS twiddle rows R + 1 entries apart
     int direct_ = 0; // radix of the direct DFT stage (0: none)
     bool local32_ = false;
     bool prefetch_ = false;

@@ -34,6 +34,9 @@ struct body_options {
     /// all columns, M = Mb): every work-item of the group's band has a column, so `active` is
     /// the band alone and work-items outside it neither load nor compute.
     bool single_tile = false;
+    /// With lds_tables: rows of R twiddles (one row per j mod p) sit R + 1 entries apart in
+    /// LDS, so work-items reading different rows at once do not share banks.
+    bool lds_table_pad = false;
 };
 /// Tile body whose input (alias_in) or output (alias_out) pointer may point into the LDS
 /// scratch passed to the body (real transforms included).

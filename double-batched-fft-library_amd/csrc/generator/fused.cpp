@@ -99,11 +99,12 @@ bool one_tile(configuration p, std::size_t kcols, device_info const &info, unsig
     return true;
 }
 
-/// Complex entries of the twiddle tables of pass c (stages 1.. and the real pre/post-pass).
-std::size_t twiddle_entries(fft1d_configuration const &c) {
+/// Complex entries of the twiddle tables of pass c (stages 1.. and the real pre/post-pass),
+/// with `pad` the rows of the stage tables one entry apart more.
+std::size_t twiddle_entries(fft1d_configuration const &c, bool pad) {
     std::size_t n = 0, p = c.factorization.empty() ? 1 : c.factorization[0];
     for (std::size_t s = 1; s < c.factorization.size(); ++s) {
-        n += p * c.factorization[s];
+        n += p * (c.factorization[s] + (pad ? 1 : 0));
         p *= c.factorization[s];
     }
     if (c.type != transform_type::c2c && c.N % 2 == 0) {
@@ -131,13 +132,16 @@ std::string fused2_configuration::identifier() const {
                    ";" + std::to_string(lds_barriers) + ";" + std::to_string(lds_tables) +
                    ";" + std::to_string(waves_per_eu) + ";" + std::to_string(copy_wave0) + ";" +
                    std::to_string(early_copy) + ";" + std::to_string(stage_swizzle_rows) + ";" +
-                   std::to_string(stage_swizzle_elems) + ";" + std::to_string(stage_load_nt));
+                   std::to_string(stage_swizzle_elems) + ";" + std::to_string(stage_load_nt) + ";" +
+                   std::to_string(lds_table_pad));
     return oss.str();
 }
 
 std::size_t fused2_configuration::lds_bytes() const {
     std::size_t const cb = 2 * static_cast<std::size_t>(a.fp);
-    std::size_t const tables = lds_tables ? (twiddle_entries(a) + twiddle_entries(b)) * cb : 0;
+    std::size_t const tables =
+        lds_tables ? (twiddle_entries(a, lds_table_pad) + twiddle_entries(b, lds_table_pad)) * cb
+                   : 0;
     return std::max({plane_elems * cb, a.lds_bytes(), b.lds_bytes()}) + tables;
 }
 
@@ -196,17 +200,27 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
     // BBFFT_FUSE_SYNC=full|lds, BBFFT_FUSE_TABLES=const|lds, BBFFT_FUSE_DMA=4|16,
     // BBFFT_FUSE_EARLY=off|on, BBFFT_FUSE_DMAWAVES=all|split, BBFFT_FUSE_WAVES=<waves per SIMD>.
     bool const r2c_first = pa.type == transform_type::r2c;
+    // LDS-only barriers and twiddle tables in LDS (rows padded, below) for real transforms;
+    // with the padded rows the c2r planes gained too (32 x 128^3 c2r 0.2027 ms against 0.2061,
+    // 2048 x 128^2 c2r 0.0557 against 0.0579), while c2c planes lost (32 x 64 x 128^2
+    // 0.2613 / 0.2618 ms against 0.2567 / 0.2573 forward / backward).
+    bool const real_plan = r2c_first || pb.type == transform_type::c2r;
     auto option = [](char const *name, char const *on, bool dflt) {
         std::string const v = env(name);
         return v.empty() ? dflt : v == on;
     };
-    fc.lds_barriers = option("BBFFT_FUSE_SYNC", "lds", r2c_first);
-    fc.lds_tables = option("BBFFT_FUSE_TABLES", "lds", r2c_first);
+    fc.lds_barriers = option("BBFFT_FUSE_SYNC", "lds", real_plan);
+    fc.lds_tables = option("BBFFT_FUSE_TABLES", "lds", real_plan);
+    // LDS twiddle rows padded to R + 1 entries (BBFFT_FUSE_TABPAD=off for A/B): the first
+    // pass's work-items t and t + 4 read rows 64 bytes apart, on the same banks. 32 x 128^3
+    // r2c 0.1974 ms against 0.1993 unpadded (processes alternated).
+    fc.lds_table_pad = fc.lds_tables && option("BBFFT_FUSE_TABPAD", "on", true);
     if (!env("BBFFT_FUSE_WAVES").empty()) {
         fc.waves_per_eu = static_cast<unsigned>(std::atoi(env("BBFFT_FUSE_WAVES").c_str()));
     }
     if (fc.lds_bytes() > info.local_memory_size) {
         fc.lds_tables = false;
+        fc.lds_table_pad = false;
     }
     if (fc.lds_bytes() > info.local_memory_size) {
         return false;
@@ -404,6 +418,7 @@ void generate_fused2(std::ostream &os, fused2_configuration const &fc, std::stri
     }
     body_options oa, ob;
     oa.lds_tables = ob.lds_tables = fc.lds_tables;
+    oa.lds_table_pad = ob.lds_table_pad = fc.lds_table_pad;
     oa.lds_barriers = ob.lds_barriers = fc.lds_barriers;
     oa.single_tile = ob.single_tile = true; // one tile per plane: Kb = K, Mb = M
     std::string const sync = fc.lds_barriers ? "BBFFT_LDS_SYNC();" : "__syncthreads();";

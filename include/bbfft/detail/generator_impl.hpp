@@ -176,6 +176,10 @@ struct BBFFT_EXPORT fused2_configuration {
     /// Twiddle tables copied into LDS once per work-group (no global loads inside the plane
     /// loop besides the staging loads).
     bool lds_tables = false;
+    /// With lds_tables: the LDS copies of the stage twiddle tables keep their rows (one per
+    /// j mod p) R + 1 entries apart, so work-items reading different rows at once do not
+    /// share LDS banks.
+    bool lds_table_pad = false;
     /// Occupancy target of the register allocator (waves per SIMD, 0: the compiler's default);
     /// 8 lets two 1024-work-item groups share a CU when their LDS fits.
     unsigned waves_per_eu = 0;

@@ -95,7 +95,8 @@ def test_fused_staged_matches_unstaged(t, M, dims, K, monkeypatch):
 
 # every variant setting off; VARIANTS override it (an empty dict: the plan's defaults)
 PLAIN = {"BBFFT_FUSE_DMA": "4", "BBFFT_FUSE_SYNC": "full", "BBFFT_FUSE_TABLES": "const",
-         "BBFFT_FUSE_EARLY": "off", "BBFFT_FUSE_DMAWAVES": "all", "BBFFT_FUSE_SWIZZLE": "off"}
+         "BBFFT_FUSE_EARLY": "off", "BBFFT_FUSE_DMAWAVES": "all", "BBFFT_FUSE_SWIZZLE": "off",
+         "BBFFT_FUSE_TABPAD": "off"}
 VARIANTS = [
     None,
     {"BBFFT_FUSE_DMA": "16", "BBFFT_FUSE_SYNC": "lds", "BBFFT_FUSE_TABLES": "lds"},
@@ -109,6 +110,7 @@ VARIANTS = [
      "BBFFT_FUSE_EARLY": "on"},
     {"BBFFT_FUSE_SYNC": "lds", "BBFFT_FUSE_TABLES": "lds", "BBFFT_FUSE_EARLY": "on",
      "BBFFT_FUSE_DMAWAVES": "split"},
+    {"BBFFT_FUSE_TABLES": "lds", "BBFFT_FUSE_TABPAD": "on"},
 ]
 
 
@@ -149,6 +151,8 @@ def test_fused_variants_match_default(t, M, dims, K, variant, monkeypatch):
         assert "BBFFT_LDS_DMA16(" in src
     if var.get("BBFFT_FUSE_TABLES") == "lds":
         assert "_tables(threadIdx.x" in src
+    if var.get("BBFFT_FUSE_TABPAD") == "on" and dims != [16, 12, 5]:
+        assert "_l[(q / " in src  # table rows padded
     if var.get("BBFFT_FUSE_DMAWAVES") == "split" and dims in ([16, 64], [8, 32, 3]):
         assert "copying waves\n" in src
     if var.get("BBFFT_FUSE_EARLY") == "on":

@@ -96,6 +96,7 @@ FUSED_VARIANTS = {
     "plain": {"BBFFT_FUSE_DMA": "4", "BBFFT_FUSE_SYNC": "full", "BBFFT_FUSE_TABLES": "const",
               "BBFFT_FUSE_EARLY": "off", "BBFFT_FUSE_DMAWAVES": "all", "BBFFT_FUSE_SWIZZLE": "off"},
     "unstaged": {"BBFFT_FUSE_STAGE": "off"},
+    "padded_tables": {"BBFFT_FUSE_TABLES": "lds", "BBFFT_FUSE_TABPAD": "on"},
 }
 
 
