@@ -195,10 +195,13 @@ bool configure_fused2(configuration const &pa, configuration const &pb, device_i
     // pass a's first stage by the waves idle in pass b, against 0.225 without the idle-wave
     // split, 0.234 without the early copy, 0.234 with 4-byte loads and 0.232 with none of
     // these; c2r (c2c first pass, all work-items busy in pass b) 0.229 ms with 16-byte loads
-    // alone, against 0.233 with 4-byte loads, 0.233-0.234 with LDS-only barriers and tables
-    // and 0.238 with the early copy as well. Environment overrides for A/B:
-    // BBFFT_FUSE_SYNC=full|lds, BBFFT_FUSE_TABLES=const|lds, BBFFT_FUSE_DMA=4|16,
-    // BBFFT_FUSE_EARLY=off|on, BBFFT_FUSE_DMAWAVES=all|split, BBFFT_FUSE_WAVES=<waves per SIMD>.
+    // alone, against 0.233 with 4-byte loads, 0.233-0.234 with LDS-only barriers and
+    // (unpadded) tables and 0.238 with the early copy as well; with padded table rows the
+    // LDS-only barriers and tables became the faster c2r setting (below). Environment
+    // overrides for A/B: BBFFT_FUSE_SYNC=full|lds, BBFFT_FUSE_TABLES=const|lds,
+    // BBFFT_FUSE_TABPAD=on|off, BBFFT_FUSE_DMA=4|16, BBFFT_FUSE_EARLY=off|on,
+    // BBFFT_FUSE_DMAWAVES=all|split|idle_a, BBFFT_FUSE_DMA_NT=on|off,
+    // BBFFT_FUSE_WAVES=<waves per SIMD>.
     bool const r2c_first = pa.type == transform_type::r2c;
     // LDS-only barriers and twiddle tables in LDS (rows padded, below) for real transforms;
     // with the padded rows the c2r planes gained too (32 x 128^3 c2r 0.2027 ms against 0.2061,
