@@ -333,6 +333,30 @@ def test_callback_source_and_language_enter_the_identifier():
     assert name(a, bb.kernel_language.hip) == name(a, bb.kernel_language.hip)
 
 
+def test_load_callback_kernels_stream_their_stores(monkeypatch):
+    """The tuning table's entry for the 32 x 128^3 plans' 128-point pass (M = 8320) stores
+    plainly; a kernel of that shape with a load callback streams its stores (flag 2 in the
+    identifier's _g field), so the callback's side data is not evicted from the last-level
+    cache by the output. A store callback does its own stores: no flag is added.
+    BBFFT_CALLBACK_NT=off keeps the entry's stores."""
+    def name(load, store):
+        cfg = bb.parse_fft_descriptor("scbo8320.128*32")
+        src = ("__device__ float2 ld(float2* in, size_t i) { return in[i]; }\n"
+               "__device__ void st(float2* o, size_t i, float2 v) { o[i] = v; }")
+        if load or store:
+            cfg.callbacks = bb.user_module(src, "ld" if load else None, "st" if store else None,
+                                           bb.kernel_language.hip)
+        return bb.kernel_configuration(cfg)["name"]
+    def flags(kernel):  # the access-hint field after the padding fields ("" when none)
+        m = re.search(r"_in0(?:_g(\d+))?", kernel)
+        return m.group(1) or ""
+    assert flags(name(False, False)) == ""
+    assert flags(name(True, False)) == "2"
+    assert flags(name(False, True)) == ""
+    monkeypatch.setenv("BBFFT_CALLBACK_NT", "off")
+    assert flags(name(True, False)) == ""
+
+
 def test_nd_decomposition_passes():
     """2-D/3-D plans run one double-batched pass per mode (reference nd_fft.hpp:61-99)."""
     cfg = bb.parse_fft_descriptor("srfo128x128x128*32")
