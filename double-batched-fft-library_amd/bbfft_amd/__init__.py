@@ -569,4 +569,6 @@ def make_plan(cfg, stream=None, cache: Optional[Cache] = None, tuning=None) -> P
     else:
         _check(_capi.load().bbfft_plan_create(ctypes.byref(cfg._to_c()), stream, ch,
                                               ctypes.byref(h)))
-    return Plan(h, cfg, stream)
+    # a tuning with half-input bits reads (re, im) half pairs: 4 bytes per input element
+    half = tuning is not None and int(tuning.get("flags", 0)) & (INPUT_F16 | INPUT_BF16)
+    return Plan(h, cfg, stream, in_elem_bytes=4 if half else None)

@@ -50,6 +50,30 @@ def test_half_input_vs_oracle(storage, mfma, M, N, K, d):
     assert oracle.rel_l2(y, ref) <= oracle.tolerance(4, N)
 
 
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_tuned_plan_with_half_input_bits(storage):
+    """make_plan with a tuning whose flags carry the half-input (and MFMA) bits, as the
+    autotuner builds its candidates: the plan takes a buffer of 4 bytes per input element and
+    computes what the half-input plan with the same tiling computes."""
+    M, N, K = 4, 512, 96
+    cfg = _cfg(M, N, K)
+    kc = bb.half_input_kernel_configuration(cfg, storage, True)
+    flags = bb.half_input_flags(storage, True)
+    tuning = {"factorization": kc["factorization"], "Nt": kc["Nt"], "Mb": kc["Mb"],
+              "Kb": kc["Kb"], "lds_pad": kc["lds_pad"], "no_pad": kc["lds_pad"] == 0,
+              "flags": flags, "waves_per_eu": 0, "groups_per_cu": 0}
+    rng = np.random.default_rng(5)
+    z = rng.uniform(-1, 1, M * N * K) + 1j * rng.uniform(-1, 1, M * N * K)
+    bits = half_util.pack(z, storage)
+    xd = torch.from_numpy(bits.view(np.int32)).to("cuda")  # exactly M N K words
+    yd = torch.empty(M * N * K, dtype=torch.complex64, device="cuda")
+    bb.make_plan(cfg, cache=bb.default_cache(), tuning=tuning).execute(xd, yd)
+    torch.cuda.synchronize()
+    ref = oracle.fft(1, cfg.shape, cfg.istride, cfg.ostride, 0, -1,
+                     half_util.unpack(bits, storage))
+    assert oracle.rel_l2(yd.cpu().numpy(), ref) <= oracle.tolerance(4, N)
+
+
 def test_half_input_launch_and_alignment():
     cfg = _cfg(4, 512, 64)
     plan = bb.make_half_input_plan(cfg, "f16", True, cache=bb.default_cache())
